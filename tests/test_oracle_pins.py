@@ -1,0 +1,201 @@
+"""Pins of the oracle (CPU restatement of propagate-tor-test_tbb.cpp).
+
+Each pin is chosen so a plausible transcription mistake (dropped term, wrong
+sign or index, transposed operand) fails at least one of them:
+  * bit-exact against the reference's own compiled routines (oracle/_ref/,
+    built from src/propagate-tor-test_acc.v3.cpp and _pstl.cpp) and against
+    golden vectors captured from them (tests/golden/),
+  * the reference program's printed summary at 10 x 9600 x 20,
+  * closed forms: glibc rand() prefix, kfact, textbook Kalman update (FP64),
+    covariance transport = J P J^T, layer-semantics invariants.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from golden.make_ref_vectors import BS, wide_block
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def _run_oracle_block(orc, kfact, par, cv, q, hits):
+    outs = []
+    for hp, hc in hits:
+        e, p = orc.propagate_block(BS, kfact, cv.ravel(), par.ravel(), q, hp.ravel())
+        ke, kp = orc.kalman_block(BS, e, p, hc.ravel(), hp.ravel())
+        outs.append(np.concatenate([p, e, kp, ke]))
+    return np.stack(outs)
+
+
+def _c0_block_inputs(orc, nlayer=8):
+    trk, hits = orc.gen_c0(2, 64, 20, BS)
+    blk = trk[:28 * BS]
+    hl = []
+    for l in range(nlayer):
+        hb = hits[l * 9 * BS:(l + 1) * 9 * BS]
+        hl.append((hb[:3 * BS].reshape(3, BS), hb[3 * BS:].reshape(6, BS)))
+    return blk[:6 * BS].reshape(6, BS), blk[6 * BS:27 * BS].reshape(21, BS), blk[27 * BS:].view(np.int32).copy(), hl
+
+
+def test_glibc_rand_prefix(orc):
+    # glibc's TYPE_3 additive generator after srand(1) (the reference never seeds)
+    assert list(orc.rand_prefix(1, 5)) == [1804289383, 846930886, 1681692777, 1714636915, 1957747793]
+
+
+def test_kfact_closed_form(orc):
+    # tbb.cpp:654: 100/(-0.299792458*3.8112) evaluated in double, narrowed to float
+    assert orc.kfact_tbb() == np.float32(100 / (-0.299792458 * 3.8112))
+    assert abs(orc.kfact_tbb() + 87.52206) < 1e-4
+
+
+def test_block_routines_match_golden_vectors(orc):
+    g = np.load(os.path.join(GOLDEN, "ref_block_vectors.npz"))
+    kfact = float(g["kfact"])
+    got_c0 = _run_oracle_block(orc, kfact, *_c0_block_inputs(orc))
+    assert np.array_equal(_bits(got_c0), _bits(g["c0"]))
+    got_w = _run_oracle_block(orc, kfact, *wide_block(int(g["wide_seed"]), 4))
+    assert np.array_equal(_bits(got_w), _bits(g["wide"]))
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_block_routines_match_reference_live(orc, ref, seed):
+    par, cv, q, hits = wide_block(seed, 6)
+    want = _run_oracle_block(orc, ref.kfact, par, cv, q, hits)
+    from golden.make_ref_vectors import run_ref_block
+    got = run_ref_block(ref, par, cv, q, hits)
+    assert np.array_equal(_bits(want), _bits(got))
+
+
+def test_generator_c0_matches_golden(orc):
+    g = np.load(os.path.join(GOLDEN, "ref_gen_c0_head.npz"))
+    nevts, ntrks, nlayer, bs = (int(v) for v in g["dims"])
+    trk, hits = orc.gen_c0(nevts, ntrks, nlayer, bs)
+    assert np.array_equal(_bits(trk[:2 * 28 * bs]), _bits(g["trk_blocks"]))
+    assert np.array_equal(_bits(hits[:2 * 9 * bs]), _bits(g["hits_layer0"]))
+
+
+def test_generator_c0_matches_reference_live(orc, ref):
+    dims, trk_r, hits_r = ref.pstl_generate()
+    trk, hits = orc.gen_c0(*dims)
+    assert np.array_equal(_bits(trk), _bits(trk_r))
+    assert np.array_equal(_bits(hits), _bits(hits_r))
+
+
+def test_c0_inputs_table_matches_reference_header(orc):
+    d = json.load(open(os.path.join(GOLDEN, "input_track_c0.json")))
+    for lay in range(20):
+        assert np.array_equal(orc.c0_hit(lay), np.float32(d["hits"][21 - lay]["pos"]))
+
+
+def test_c0_summary_matches_reference_program(orc):
+    import oracle
+    trk, hits = orc.gen_c0(10, 9600, 20, BS)
+    out = orc.run_f32(trk, hits, 10 * 300, 20, BS, semantics=0)
+    st, cn = orc.summary(out, 10, 9600, BS, orc.c0_hit(19))
+    want = open(os.path.join(GOLDEN, "c0_summary_ref.txt")).read().split("\n")[:13]
+    assert oracle.summary_lines(st, cn) == want
+
+
+def _sym_full(c):
+    P = np.zeros((6, 6))
+    k = 0
+    for i in range(6):
+        for j in range(i + 1):
+            P[i, j] = P[j, i] = c[k]
+            k += 1
+    return P
+
+
+def _sym_pack(P):
+    return np.array([P[i, j] for i in range(6) for j in range(i + 1)])
+
+
+def _random_pd(rng, n, scale):
+    A = rng.standard_normal((n, n))
+    return (A @ A.T + n * np.eye(n)) * scale
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_kalman_update_is_textbook_fp64(orc, seed):
+    """K = P H^T S^-1, S = H P H^T + R V R^T, H = R [I3 0], R rows (r00, r01, 0), (0, 0, 1)."""
+    rng = np.random.default_rng(seed)
+    P = _random_pd(rng, 6, 1e-4)
+    Vf = _random_pd(rng, 3, 1e-5)
+    par = np.array([-30.0, -20.0, 10.0, 0.3, -2.5, 0.9]) + rng.standard_normal(6) * 0.1
+    hit = np.array([-31.0, -19.5, 10.3])
+    V = np.array([Vf[0, 0], Vf[1, 0], Vf[1, 1], Vf[2, 0], Vf[2, 1], Vf[2, 2]])
+    p_out, c_out, chi2 = orc.kalman_lane_f64(par, _sym_pack(P), hit, V)
+    r = np.hypot(hit[0], hit[1])
+    Rm = np.array([[-(hit[1] + par[1]) / (2 * r), (hit[0] + par[0]) / (2 * r), 0], [0, 0, 1]])
+    H = np.hstack([Rm, np.zeros((2, 3))])
+    S = H @ P @ H.T + Rm @ Vf @ Rm.T
+    K = P @ H.T @ np.linalg.inv(S)
+    res = Rm @ (hit - par[:3])
+    np.testing.assert_allclose(p_out, par + K @ res, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(_sym_full(c_out), P - K @ H @ P, rtol=1e-9, atol=1e-16)
+    np.testing.assert_allclose(chi2, res @ np.linalg.inv(S) @ res, rtol=1e-10)
+
+
+@pytest.mark.parametrize("layer", [0, 7, 19])
+def test_transport_is_J_P_Jt_fp64(orc, layer):
+    d = json.load(open(os.path.join(GOLDEN, "input_track_c0.json")))
+    par = np.array(d["track"]["par"])
+    cov = np.array(d["track"]["cov"]) * 100
+    hit = np.array(d["hits"][21 - layer]["pos"])
+    p, c, J = orc.propagate_lane_f64(par, cov, 1, hit, float(orc.kfact_tbb()))
+    np.testing.assert_allclose(_sym_full(c), J @ _sym_full(cov) @ J.T, rtol=1e-12, atol=1e-20)
+    # structure of errorProp (tbb.cpp:753-807): rows 3 and 5 are unit rows,
+    # columns 2 and 5 vanish outside rows 2 and their diagonals
+    assert np.array_equal(J[3], np.eye(6)[3]) and np.array_equal(J[5], np.eye(6)[5])
+    assert J[0, 2] == J[0, 5] == J[1, 2] == J[1, 5] == J[4, 2] == J[4, 5] == 0.0
+    assert J[2, 2] == 1.0
+    # the propagated point lies on the hit's cylinder (5 fixed-point steps)
+    assert abs(np.hypot(p[0], p[1]) - np.hypot(hit[0], hit[1])) < 1e-6 * np.hypot(hit[0], hit[1])
+    # pt, theta unchanged; phi advances by the turning angle
+    assert p[3] == par[3] and p[5] == par[5]
+
+
+def test_reference_semantics_is_last_layer_only(orc):
+    """tbb.cpp:894 restarts every layer from the input: output = input updated with the last layer only."""
+    trk, hits = orc.gen_c0(1, 64, 20, BS)
+    nblk = 2
+    full = orc.run_f32(trk, hits, nblk, 20, BS, semantics=0)
+    h = hits.reshape(nblk, 20, 9 * BS)
+    last = np.ascontiguousarray(h[:, 19:20, :]).ravel()
+    one = orc.run_f32(trk, last, nblk, 1, BS, semantics=0)
+    assert np.array_equal(_bits(full), _bits(one))
+
+
+def test_chained_semantics_is_fold_of_single_layers(orc):
+    """propagate-tor-test_hip.cu:839-845: layer L starts from layer L-1's updated state."""
+    trk, hits = orc.gen_c0(1, 64, 6, BS)
+    nblk = 2
+    full = orc.run_f32(trk, hits, nblk, 6, BS, semantics=1)
+    h = hits.reshape(nblk, 6, 9 * BS)
+    cur = trk
+    for l in range(6):
+        cur = orc.run_f32(cur, np.ascontiguousarray(h[:, l:l + 1, :]).ravel(), nblk, 1, BS, semantics=1)
+    assert np.array_equal(_bits(full), _bits(cur))
+
+
+def test_fp64_replay_close_to_fp32(orc):
+    trk, hits = orc.gen_c0(1, 64, 20, BS)
+    out = orc.run_f32(trk, hits, 2, 20, BS, semantics=0)
+    par64, cov64, _ = orc.run_f64(trk, hits, 2, 20, BS, semantics=0)
+    o = out.reshape(2, 28, BS)
+    par32 = o[:, :6, :].transpose(0, 2, 1).reshape(-1, 6)
+    np.testing.assert_allclose(par32, par64, rtol=1e-5)
+
+
+def test_opcount_frozen(orc):
+    """FLOPs per track-layer update of the reference algorithm (propagateToR + KalmanUpdate)."""
+    c = orc.opcount()
+    assert c == {"add": 494, "mul": 757, "div": 12, "sqrt": 8, "trig": 4, "trivial": 63, "chi2": 9}
+    total = c["add"] + c["mul"] + c["div"] + c["sqrt"] + c["trig"]
+    assert total == 1275 and total - c["trivial"] == 1212
