@@ -1,0 +1,117 @@
+/*
+ * p2r on MI355X -- C-ABI of the propagate-to-radius + Kalman-update hot path.
+ *
+ * Data layout (byte-identical to the reference's Matriplex,
+ * cerati/p2r-tests src/propagate-tor-test_tbb.cpp:68-125), bs = bsize:
+ *   MPTRK block: float par[6*bs] | float cov[21*bs] | int32 q[bs]   (112*bs bytes)
+ *   MPHIT block: float pos[3*bs] | float cov[6*bs]                  ( 36*bs bytes)
+ *   field f of lane it at data[f*bs + it]; par = (x, y, z, 1/pt, phi, theta);
+ *   cov packed lower-triangular symmetric, (i,j) i>=j at i*(i+1)/2 + j.
+ *   Uniform batch: track block (ev, ib) at index ib + nb*ev, nb = ntrks/bs
+ *   (tbb.cpp:147-153); hit block (ev, ib, layer) at layer + nlayer*(ib + nb*ev)
+ *   (tbb.cpp:215-217).  A flat track index is block*bs + lane (= ev*ntrks + tk).
+ *
+ * Conventions: every pointer passed to an entry point that takes a `stream` is
+ * device memory; the caller owns all buffers (as in the reference driver);
+ * calls enqueue on `stream` (a hipStream_t, NULL = default stream) and do not
+ * synchronize.  Every entry point returns a p2r_status and never aborts.
+ */
+#ifndef P2R_H
+#define P2R_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum p2r_status {
+  P2R_OK = 0,
+  P2R_ERR_ARG = 1,         /* null pointer / invalid enum value */
+  P2R_ERR_SHAPE = 2,       /* sizes the path does not accept (e.g. ntrks % bsize != 0) */
+  P2R_ERR_HIP = 3,         /* a HIP runtime call failed; see p2r_last_error */
+  P2R_ERR_UNSUPPORTED = 4, /* valid request this build does not implement */
+  P2R_ERR_NOMEM = 5        /* workspace allocation failed */
+} p2r_status;
+
+/* Layer semantics.
+ * P2R_REFERENCE: every layer restarts from the input track and the output is
+ *   the last layer's update -- the reference CPU path (tbb.cpp:880-911).
+ * P2R_CHAINED: layer L starts from layer L-1's updated state -- the in-place
+ *   GPU variant (propagate-tor-test_hip.cu:839-845), a real Kalman filter. */
+typedef enum p2r_semantics { P2R_REFERENCE = 0, P2R_CHAINED = 1 } p2r_semantics;
+
+typedef struct p2r_ctx p2r_ctx;
+
+/* Batch shape: the reference's compile-time nevts, ntrks, nlayer, bsize
+ * (tbb.cpp:15-34), here runtime values.  bsize in [1, 1024]. */
+typedef struct p2r_shape {
+  int32_t nevts;
+  int32_t ntrks;
+  int32_t nlayer;
+  int32_t bsize;
+} p2r_shape;
+
+/* Arithmetic of the per-lane math.
+ * P2R_EXACT: every FP32 operation rounds as the reference's source does (no
+ *   fused multiply-add, IEEE division and sqrt, FP64 determinant, sin/cos
+ *   rounded from FP64); bit-identical to the IEEE-strict CPU build of the
+ *   reference algorithm wherever the CPU's sinf/cosf are correctly rounded.
+ * P2R_FAST: fused multiply-adds, hardware reciprocal/sqrt and float sincos;
+ *   same algorithm, different FP32 rounding. */
+typedef enum p2r_precision { P2R_EXACT = 0, P2R_FAST = 1 } p2r_precision;
+
+/* kfact: curvature constant, 100/(-0.299792458*Bz) in the reference
+ * (tbb.cpp:654, Bz = 3.8112 T -> -87.52206); b_scale multiplies the field:
+ * the kernel uses kfact / b_scale.  p2r_default_params() gives the reference's
+ * constant, b_scale 1 and P2R_EXACT. */
+typedef struct p2r_params {
+  float kfact;
+  float b_scale;
+  int32_t precision;
+} p2r_params;
+
+/* Context: device binding and workspace.  Replaces the reference's
+ * p2r_get_compute_device_id / p2r_get_streams set-up
+ * (propagate-tor-test_hip_local.cu:112-136). */
+int p2r_create(int device, p2r_ctx** ctx);
+int p2r_destroy(p2r_ctx* ctx);
+p2r_params p2r_default_params(void);
+const char* p2r_error_string(int status);
+/* Detail text of the last failure on this context (never NULL). */
+const char* p2r_last_error(const p2r_ctx* ctx);
+
+/* Whole batch, all layers in one kernel: the reference's layer loop
+ * (tbb.cpp:873-925 per repetition; GPU twins launch_p2r_kernel,
+ * propagate-tor-test_hip_local.cu:911-939, and GPUsequence,
+ * propagate-tor-test_hip.cu:820-849).  trk_in / trk_out: nevts*nb MPTRK
+ * blocks; hits: nlayer*nevts*nb MPHIT blocks.  trk_out may equal trk_in.
+ * chi2 (nullable): per-track sum over layers of r^T S^-1 r (the local chi2
+ * the reference leaves commented out, tbb.cpp:631), float[nevts*ntrks] by flat
+ * track index.  Output q equals input q (the reference copies it, tbb.cpp:880). */
+int p2r_forward(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* shape,
+                int semantics, const p2r_params* params, float* chi2, void* stream);
+
+/* Block-level mirrors of the reference entry points, over nblk Matriplex
+ * blocks.  Pointers address the field arrays of the first block; consecutive
+ * blocks are `trk_stride` floats apart for track-side arrays and `hit_stride`
+ * floats apart for hit-side arrays (e.g. an MPTRK array: trk_stride = 28*bs;
+ * layer L of a uniform hit array: msP = hits + L*9*bs, hit_stride = nlayer*9*bs).
+ *
+ * propagateToR(inErr, inPar, inChg, msP, outErr, outPar), tbb.cpp:656-657.
+ * Output arrays use the same stride as the inputs.  Both block-level entry
+ * points use P2R_EXACT arithmetic. */
+int p2r_propagateToR(p2r_ctx* ctx, const float* inErr, const float* inPar, const int32_t* inChg, const float* msP,
+                     float* outErr, float* outPar, int32_t bsize, int64_t nblk, int64_t trk_stride,
+                     int64_t hit_stride, float kfact, void* stream);
+
+/* KalmanUpdate(trkErr, inPar, hitErr, msP), in place, tbb.cpp:433. */
+int p2r_KalmanUpdate(p2r_ctx* ctx, float* trkErr, float* inPar, const float* hitErr, const float* msP,
+                     int32_t bsize, int64_t nblk, int64_t trk_stride, int64_t hit_stride, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* P2R_H */

@@ -52,6 +52,7 @@ class Oracle:
         L.orc_rand_prefix.argtypes = [C.c_uint, i, vp]
         L.orc_summary.argtypes = [vp, i, i, i, f, f, f, vp, vp]
         L.orc_opcount.argtypes = [vp]
+        L.orc_set_trig_mode.argtypes = [i]
 
     # -- block-level mirrors of the reference entry points --
     def propagate_block(self, bs, kfact, inErr, inPar, inChg, msP):
@@ -84,13 +85,17 @@ class Oracle:
         return par, cov, float(c2[0])
 
     # -- whole batches in the reference Matriplex layout --
-    def run_f32(self, trk, hits, nblk, nlayer, bs, semantics=0, kfact=None, nthreads=0, chi2=False, mask=None):
+    def run_f32(self, trk, hits, nblk, nlayer, bs, semantics=0, kfact=None, nthreads=0, chi2=False, mask=None,
+                cr_trig=False):
+        """cr_trig: sin/cos correctly rounded (FP64 rounded to float) instead of libm's sinf/cosf."""
         kfact = self.kfact_tbb() if kfact is None else kfact
+        self.lib.orc_set_trig_mode(1 if cr_trig else 0)
         out = np.zeros_like(trk)
         c2 = np.zeros(nblk * bs, np.float32) if chi2 else None
         m = None if mask is None else np.ascontiguousarray(mask, np.uint32)
         rc = self.lib.orc_run_f32(_ptr(trk), _ptr(hits), _ptr(out), nblk, nlayer, bs, semantics, kfact, nthreads,
                                   _ptr(c2), _ptr(m))
+        self.lib.orc_set_trig_mode(0)
         assert rc == 0
         return (out, c2) if chi2 else out
 

@@ -161,6 +161,9 @@ void orc_kalman_lane_f64(double par[6], double cov[21], const double hitpos[3], 
 
 // ---- full runs --------------------------------------------------------------
 
+// Selects the FP32 path's sin/cos (see trig_mode in p2r_oracle.hpp).
+void orc_set_trig_mode(int mode) { trig_mode() = mode ? 1 : 0; }
+
 // FP32 run over nblk Matriplex blocks in the reference layout (the driver loop
 // of tbb.cpp:873-925 for one repetition).  `nblk` = nevts*nb.  chi2 (nullable)
 // receives per-track sum of local chi2 over the processed layers, indexed by

@@ -43,9 +43,15 @@ inline constexpr int sym3(int i, int j) { return sym6(i, j); }
 // Scalar helpers, overloaded per scalar type.
 inline float o_sqrt(float x) { return std::sqrt(x); }
 inline double o_sqrt(double x) { return std::sqrt(x); }
-inline float o_cos(float x) { return std::cos(x); }
+// Trig of the FP32 path: 0 = libm cosf/sinf as the reference calls them
+// (tbb.cpp:689, :768-769); 1 = the FP64 value rounded to float (correctly
+// rounded), the variant the GPU's exact arithmetic is checked against
+// bit-for-bit.  glibc's sinf/cosf differ from the correctly rounded value on
+// ~0.1% of arguments; nothing else differs between the two modes.
+inline int& trig_mode() { static int m = 0; return m; }
+inline float o_cos(float x) { return trig_mode() ? (float)std::cos((double)x) : std::cos(x); }
 inline double o_cos(double x) { return std::cos(x); }
-inline float o_sin(float x) { return std::sin(x); }
+inline float o_sin(float x) { return trig_mode() ? (float)std::sin((double)x) : std::sin(x); }
 inline double o_sin(double x) { return std::sin(x); }
 inline double o_val(float x) { return x; }
 inline double o_val(double x) { return x; }

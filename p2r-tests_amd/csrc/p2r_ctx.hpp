@@ -1,0 +1,59 @@
+// Context object behind the opaque p2r_ctx handle of include/p2r.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+
+#include "../../include/p2r.h"
+
+// Makes `device` current for the scope and restores the caller's device.
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int device) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != device) (void)hipSetDevice(device);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+struct p2r_ctx {
+  int device = 0;
+  int num_cus = 0;
+  char msg[256] = "no error";
+  // Workspace owned by the context (device memory), grown on demand.
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+
+  int fail(int status, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(msg, sizeof(msg), fmt, ap);
+    va_end(ap);
+    return status;
+  }
+  int check_launch(const char* what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(P2R_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    return P2R_OK;
+  }
+  // Ensures at least `bytes` of workspace; returns nullptr on failure.
+  void* workspace(size_t bytes) {
+    if (bytes <= ws_bytes) return ws;
+    release_workspace();
+    if (hipMalloc(&ws, bytes) != hipSuccess) {
+      ws = nullptr;
+      return nullptr;
+    }
+    ws_bytes = bytes;
+    return ws;
+  }
+  void release_workspace() {
+    if (ws) (void)hipFree(ws);
+    ws = nullptr;
+    ws_bytes = 0;
+  }
+};

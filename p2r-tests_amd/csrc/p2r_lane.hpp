@@ -1,0 +1,86 @@
+// Per-lane p2r math for gfx950: one track per lane, all state in VGPRs.
+//
+// The arithmetic is the reference's (cerati/p2r-tests,
+// src/propagate-tor-test_tbb.cpp): helix propagation to the hit radius with 5
+// fixed-point steps and a 4th-order Taylor sin/cos of the turning angle
+// (tbb.cpp:656-808), covariance transport P' = J P J^T (tbb.cpp:296-373) and
+// the tangent-plane Kalman update with an FP64 2x2 determinant
+// (tbb.cpp:433-642).  Formulas and their evaluation order follow those lines
+// so results agree with the reference's FP32 CPU path to a few ulps (the
+// exact instance rounds every operation as the FP32 source does; the fast
+// instance fuses multiply-adds and uses the hardware reciprocal and sqrt).
+//
+// Differences in form, not in value: the transport skips the Jacobian's
+// structural zeros and ones, including row 3 (a unit row) which the reference
+// multiplies out generically (tbb.cpp:322-327, 357-360); for finite inputs the
+// result is identical.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace p2r {
+
+constexpr int kNiter = 5;  // fixed-point steps of propagateToR, tbb.cpp:655
+
+struct Trk {
+  float par[6];   // x, y, z, 1/pt, phi, theta
+  float cov[21];  // packed lower-triangular symmetric
+};
+
+struct Hit {
+  float pos[3];
+  float cov[6];  // xx, yx, yy, zx, zy, zz
+};
+
+__host__ __device__ constexpr int sym(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
+
+// Keeps `v` opaque to the optimizer (no code).  Used so values that are
+// loop-invariant only because of reference semantics are recomputed per layer
+// rather than hoisted: each layer then does the full work it is credited with.
+__device__ __forceinline__ void opaque(float& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void opaque(Trk& t) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) opaque(t.par[i]);
+#pragma unroll
+  for (int i = 0; i < 21; ++i) opaque(t.cov[i]);
+}
+
+// Non-trivial entries of the propagation Jacobian (errorProp, tbb.cpp:753-807).
+// Rows 3 and 5 are unit rows; J22 = 1; J02 = J05 = J12 = J15 = J42 = J45 = 0.
+struct Jac {
+  float j00, j01, j03, j04;
+  float j10, j11, j13, j14;
+  float j20, j21, j23, j24, j25;
+  float j40, j41, j43, j44;
+};
+
+// Two instances of the per-lane math (see p2r_lane_math.inc).
+namespace exact {
+#pragma clang fp contract(off)
+#define P2R_EXACT 1
+#include "p2r_lane_math.inc"
+#undef P2R_EXACT
+}  // namespace exact
+
+namespace fast {
+#pragma clang fp contract(fast)
+#define P2R_EXACT 0
+#include "p2r_lane_math.inc"
+#undef P2R_EXACT
+}  // namespace fast
+
+// Compile-time selection of the instance.
+template <bool EXACT> struct Lane;
+template <> struct Lane<true> {
+  static __device__ __forceinline__ void propagate(const Trk& in, float k, float hx, float hy, Trk& out) {
+    exact::propagate_to_r(in, k, hx, hy, out);
+  }
+  static __device__ __forceinline__ float update(Trk& t, const Hit& h) { return exact::kalman_update(t, h); }
+};
+template <> struct Lane<false> {
+  static __device__ __forceinline__ void propagate(const Trk& in, float k, float hx, float hy, Trk& out) {
+    fast::propagate_to_r(in, k, hx, hy, out);
+  }
+  static __device__ __forceinline__ float update(Trk& t, const Hit& h) { return fast::kalman_update(t, h); }
+};
+
+}  // namespace p2r

@@ -1,0 +1,168 @@
+"""p2r on MI355X: Python host side of the propagate-to-radius + Kalman-update path.
+
+The compute runs in lib/libp2r_amd.so (hand-written HIP for gfx950) behind the
+C-ABI of include/p2r.h; this package binds it with ctypes, passes device
+pointers of torch tensors (torch provides device memory and streams) and
+mirrors the reference's entry points (cerati/p2r-tests
+src/propagate-tor-test_tbb.cpp: propagateToR :656, KalmanUpdate :433, the
+driver loop :873-925).
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._capi import (P2R_CHAINED, P2R_EXACT, P2R_FAST, P2R_ERR_ARG, P2R_ERR_HIP, P2R_ERR_NOMEM, P2R_ERR_SHAPE, P2R_ERR_UNSUPPORTED,
+                    P2R_OK, P2R_REFERENCE, P2RError, lib, p2r_gen_config, p2r_params, p2r_shape)
+from .matriplex import Shape, tracks_to_arrays, hits_block_view  # noqa: F401
+
+SEMANTICS = {"reference": P2R_REFERENCE, "chained": P2R_CHAINED}
+
+
+def _ptr(t):
+    """Device (torch) or host (numpy) buffer -> void*; None -> NULL."""
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data_as(C.c_void_p)
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return C.c_void_p(s.cuda_stream)
+
+
+PRECISION = {"exact": P2R_EXACT, "fast": P2R_FAST}
+
+
+def default_params(precision="exact", b_scale=1.0):
+    p = lib().p2r_default_params()
+    prec = PRECISION[precision] if isinstance(precision, str) else int(precision)
+    return p2r_params(p.kfact, float(b_scale), prec)
+
+
+class Context:
+    """A device binding (p2r_create / p2r_destroy)."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        st = lib().p2r_create(int(device), C.byref(self._h))
+        if st != P2R_OK:
+            raise P2RError(st, f"p2r_create(device={device})")
+        self.device = device
+
+    def close(self):
+        if self._h:
+            lib().p2r_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what):
+        if st != P2R_OK:
+            raise P2RError(st, f"{what}: {lib().p2r_last_error(self._h).decode()}")
+
+    def forward(self, trk, hits, out, shape, semantics="reference", params=None, chi2=None, stream=None,
+                precision=None):
+        """Whole batch through all layers (reference layout; torch CUDA float32 tensors)."""
+        sem = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
+        sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
+        p = params if params is not None else default_params(precision or "exact")
+        st = lib().p2r_forward(self._h, _ptr(trk), _ptr(hits), _ptr(out), C.byref(sh), sem, C.byref(p), _ptr(chi2),
+                               _stream(stream))
+        self._check(st, "p2r_forward")
+
+    def propagateToR(self, inErr, inPar, inChg, msP, outErr, outPar, bsize, nblk, trk_stride, hit_stride,
+                     kfact=None, stream=None):
+        """Mirror of propagateToR (tbb.cpp:656) over nblk blocks; pointers/tensors address field arrays."""
+        k = default_params().kfact if kfact is None else kfact
+        st = lib().p2r_propagateToR(self._h, _ptr(inErr), _ptr(inPar), _ptr(inChg), _ptr(msP), _ptr(outErr),
+                                    _ptr(outPar), bsize, nblk, trk_stride, hit_stride, k, _stream(stream))
+        self._check(st, "p2r_propagateToR")
+
+    def KalmanUpdate(self, trkErr, inPar, hitErr, msP, bsize, nblk, trk_stride, hit_stride, stream=None):
+        """Mirror of KalmanUpdate (tbb.cpp:433), in place, over nblk blocks."""
+        st = lib().p2r_KalmanUpdate(self._h, _ptr(trkErr), _ptr(inPar), _ptr(hitErr), _ptr(msP), bsize, nblk,
+                                    trk_stride, hit_stride, _stream(stream))
+        self._check(st, "p2r_KalmanUpdate")
+
+
+# ---- host generators --------------------------------------------------------
+
+def gen_reference(shape):
+    """Reference-compatible inputs (tbb.cpp:243-293 + input_track.h) as host float32 arrays."""
+    trk = np.zeros(shape.trk_floats, np.float32)
+    hits = np.zeros(shape.hit_floats, np.float32)
+    st = lib().p2r_gen_reference(_ptr(trk), _ptr(hits), C.byref(p2r_shape(*shape.tuple())))
+    if st != P2R_OK:
+        raise P2RError(st, "p2r_gen_reference")
+    return trk, hits
+
+
+def reference_hit(layer):
+    p = np.zeros(3, np.float32)
+    st = lib().p2r_reference_hit(int(layer), _ptr(p))
+    if st != P2R_OK:
+        raise P2RError(st, "p2r_reference_hit")
+    return p
+
+
+def gen_config(**kw):
+    c = lib().p2r_gen_default_config()
+    offsets = kw.pop("layer_offsets", None)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    if offsets is not None:
+        offsets = np.ascontiguousarray(offsets, np.float32)
+        c.layer_offsets = offsets.ctypes.data
+        c._keep = offsets  # keep the array alive with the struct
+    return c
+
+
+def gen_uniform(shape, cfg=None):
+    """Seeded BASELINE-style inputs (include/p2r_gen.h) as host float32 arrays."""
+    cfg = cfg if cfg is not None else gen_config()
+    trk = np.zeros(shape.trk_floats, np.float32)
+    hits = np.zeros(shape.hit_floats, np.float32)
+    st = lib().p2r_gen_uniform(_ptr(trk), _ptr(hits), C.byref(p2r_shape(*shape.tuple())), C.byref(cfg))
+    if st != P2R_OK:
+        raise P2RError(st, "p2r_gen_uniform")
+    return trk, hits
+
+
+def lognormal_counts(nevts, median=9600, sigma=0.5, lo=500, hi=40000, seed=0x50325221):
+    out = np.zeros(nevts, np.int32)
+    st = lib().p2r_gen_lognormal_counts(_ptr(out), nevts, median, sigma, lo, hi, seed)
+    if st != P2R_OK:
+        raise P2RError(st, "p2r_gen_lognormal_counts")
+    return out
+
+
+def ragged_layout(counts, bsize):
+    counts = np.ascontiguousarray(counts, np.int32)
+    offs = np.zeros(len(counts) + 1, np.int64)
+    nblk = C.c_int64()
+    st = lib().p2r_gen_ragged_layout(_ptr(counts), len(counts), bsize, _ptr(offs), C.byref(nblk))
+    if st != P2R_OK:
+        raise P2RError(st, "p2r_gen_ragged_layout")
+    return offs, int(nblk.value)
+
+
+def gen_ragged(counts, nlayer, bsize, kmin=5, kmax=20, cfg=None):
+    """Uneven batch: returns (trk, hits, layer_mask, blk_offsets) host arrays."""
+    counts = np.ascontiguousarray(counts, np.int32)
+    offs, nblk = ragged_layout(counts, bsize)
+    cfg = cfg if cfg is not None else gen_config()
+    trk = np.zeros(nblk * 28 * bsize, np.float32)
+    hits = np.zeros(nblk * nlayer * 9 * bsize, np.float32)
+    mask = np.zeros(nblk * bsize, np.uint32)
+    st = lib().p2r_gen_ragged(_ptr(trk), _ptr(hits), _ptr(mask), _ptr(counts), len(counts), _ptr(offs), nlayer, bsize,
+                              kmin, kmax, C.byref(cfg))
+    if st != P2R_OK:
+        raise P2RError(st, "p2r_gen_ragged")
+    return trk, hits, mask, offs

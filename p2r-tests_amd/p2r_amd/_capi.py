@@ -1,0 +1,70 @@
+"""ctypes binding of lib/libp2r_amd.so (C-ABI declared in include/p2r.h, include/p2r_gen.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libp2r_amd.so")
+
+P2R_OK, P2R_ERR_ARG, P2R_ERR_SHAPE, P2R_ERR_HIP, P2R_ERR_UNSUPPORTED, P2R_ERR_NOMEM = range(6)
+P2R_REFERENCE, P2R_CHAINED = 0, 1
+P2R_EXACT, P2R_FAST = 0, 1
+
+
+class p2r_shape(C.Structure):
+    _fields_ = [("nevts", C.c_int32), ("ntrks", C.c_int32), ("nlayer", C.c_int32), ("bsize", C.c_int32)]
+
+
+class p2r_params(C.Structure):
+    _fields_ = [("kfact", C.c_float), ("b_scale", C.c_float), ("precision", C.c_int32)]
+
+
+class p2r_gen_config(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("seed", C.c_uint64), ("smear", C.c_float), ("r0", C.c_float), ("dr", C.c_float),
+                ("hit_var", C.c_float), ("layer_offsets", C.c_void_p), ("b_scale_truth", C.c_float)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"p2r HIP library not built: {LIB_PATH} (run __graft_entry__.build() or "
+                               f"`make -C p2r-tests_amd`)")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float
+        sig = {
+            "p2r_create": ([C.c_int, C.POINTER(vp)], C.c_int),
+            "p2r_destroy": ([vp], C.c_int),
+            "p2r_default_params": ([], p2r_params),
+            "p2r_error_string": ([C.c_int], C.c_char_p),
+            "p2r_last_error": ([vp], C.c_char_p),
+            "p2r_forward": ([vp, vp, vp, vp, C.POINTER(p2r_shape), C.c_int, C.POINTER(p2r_params), vp, vp], C.c_int),
+            "p2r_propagateToR": ([vp, vp, vp, vp, vp, vp, vp, i32, i64, i64, i64, f32, vp], C.c_int),
+            "p2r_KalmanUpdate": ([vp, vp, vp, vp, vp, i32, i64, i64, i64, vp], C.c_int),
+            "p2r_gen_reference": ([vp, vp, C.POINTER(p2r_shape)], C.c_int),
+            "p2r_reference_hit": ([C.c_int, vp], C.c_int),
+            "p2r_gen_default_config": ([], p2r_gen_config),
+            "p2r_gen_uniform": ([vp, vp, C.POINTER(p2r_shape), C.POINTER(p2r_gen_config)], C.c_int),
+            "p2r_gen_ragged_layout": ([vp, i32, i32, vp, vp], C.c_int),
+            "p2r_gen_ragged": ([vp, vp, vp, vp, i32, vp, i32, i32, i32, i32, C.POINTER(p2r_gen_config)], C.c_int),
+            "p2r_gen_lognormal_counts": ([vp, i32, C.c_double, C.c_double, i32, i32, C.c_uint64], C.c_int),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+class P2RError(RuntimeError):
+    def __init__(self, status, detail=""):
+        self.status = status
+        msg = lib().p2r_error_string(status).decode()
+        super().__init__(f"p2r status {status} ({msg}){': ' + detail if detail else ''}")
