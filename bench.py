@@ -50,6 +50,13 @@ def main():
     ap.add_argument("--bsize", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true", help="time only the headline variant")
+    ap.add_argument("--no-ragged", action="store_true", help="skip the uneven-batch (config 3) measurement")
+    ap.add_argument("--ragged-events", type=int, default=500)
+    ap.add_argument("--no-parity", action="store_true", help="skip the parity spot check")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (steps times) and exit: the "
+                         "reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -62,6 +69,17 @@ def main():
 
     import p2r_amd
     from p2r_amd import Shape
+
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
+                          "launches": args.steps}), flush=True)
+        return
 
     shape = Shape(args.nevts, args.ntrks, args.nlayer, args.bsize)
     t0 = time.time()
@@ -94,7 +112,7 @@ def main():
 
     # the other semantics / precision, same inputs (not the headline)
     variants = []
-    for sem in ("reference", "chained"):
+    for sem in (() if args.no_variants else ("reference", "chained")):
         for prec in ("exact", "fast"):
             if (sem, prec) == (args.semantics, args.precision):
                 continue
@@ -114,6 +132,8 @@ def main():
     # parity spot check of this run's output on the first event against the oracle
     parity_res = None
     try:
+        if args.no_parity:
+            raise RuntimeError("skipped (--no-parity)")
         import oracle
         import parity
         step()
@@ -137,6 +157,13 @@ def main():
                           "fields_failing": r["fields_failing"]}
     except Exception as ex:  # the bench line is still printed; parity is covered by the tests
         parity_res = {"error": repr(ex)}
+
+    ragged = None
+    if not args.no_ragged:
+        del d_trk, d_hit, d_out
+        torch.cuda.empty_cache()
+        ragged = ragged_bench(ctx, args, {v["precision"]: v for v in variants if v["semantics"] == "chained"},
+                              (args.semantics, args.precision, value))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -170,11 +197,66 @@ def main():
                      "kernel": "fwd_uniform_kernel"},
         "cpu_baseline": cpu,
         "variants": variants,
+        "ragged": ragged,
         "parity": parity_res,
         "wall_s_timed": wall,
         "gen_s": gen_s,
     }
     print(json.dumps(line), flush=True)
+
+
+def ragged_bench(ctx, args, uniform_chained, headline):
+    """events with lognormal track counts (median 9600, sigma 0.5, clipped to
+    [500, 40000]), hit count per track uniform in 5..20 (random subset of the 20 layers), chained
+    semantics, one persistent launch replayed from a captured HIP graph.  Rate = present-layer updates/s,
+    compared with the uniform batch of the headline shape in chained semantics."""
+    import numpy as np
+    import torch
+
+    import p2r_amd
+    t0 = time.time()
+    counts = p2r_amd.lognormal_counts(args.ragged_events, median=9600, sigma=0.5, lo=500, hi=40000)
+    trk, hits, mask, offs = p2r_amd.gen_ragged(counts, 20, args.bsize, kmin=5, kmax=20)
+    gen_s = time.time() - t0
+    updates = int(np.unpackbits(mask.view(np.uint8)).sum())
+    dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
+    dm = torch.from_numpy(mask.view(np.int32)).cuda()
+    do = torch.empty_like(dt)
+    del trk, hits
+    stream = torch.cuda.Stream()
+    res = {"events": args.ragged_events, "tracks": int(counts.sum()), "slots": int(mask.size),
+           "present_updates": updates, "gen_s": gen_s, "tile_rounds": {}}
+    for prec in ("exact", "fast"):
+        best = None
+        for rounds in (1, 2, 4):
+            with torch.cuda.stream(stream):
+                for _ in range(2):
+                    ctx.forward_ragged(dt, dh, do, offs, dm, 20, args.bsize, "chained", precision=prec,
+                                       tile_rounds=rounds, stream=stream)
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=stream):
+                    ctx.forward_ragged(dt, dh, do, offs, dm, 20, args.bsize, "chained", precision=prec,
+                                       tile_rounds=rounds, stream=stream)
+                g.replay()
+                stream.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(5):
+                    g.replay()
+                e1.record(stream)
+            stream.synchronize()
+            ms = e0.elapsed_time(e1) / 5
+            rate = updates / (ms * 1e-3)
+            res["tile_rounds"][f"{prec}_r{rounds}"] = {"ms": ms, "updates_per_s": rate}
+            if best is None or rate > best[1]:
+                best = (rounds, rate, ms)
+        uni = uniform_chained.get(prec, {}).get("value")
+        if uni is None and headline[0] == "chained" and headline[1] == prec:
+            uni = headline[2]
+        res[prec] = {"best_tile_rounds": best[0], "ms": best[2], "updates_per_s": best[1],
+                     "uniform_chained_updates_per_s": uni,
+                     "fraction_of_uniform": (best[1] / uni) if uni else None}
+    return res
 
 
 def measured_traffic():

@@ -93,6 +93,55 @@ const char* p2r_last_error(const p2r_ctx* ctx);
 int p2r_forward(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* shape,
                 int semantics, const p2r_params* params, float* chi2, void* stream);
 
+/* Uneven batch in one persistent launch.  Events of different sizes, each
+ * padded to whole Matriplex blocks: event e owns blocks
+ * [blk_offsets[e], blk_offsets[e+1]) (nevts+1 offsets, blk_offsets[0] = 0,
+ * non-decreasing); a flat slot index is block*bsize + lane.  hits: nlayer
+ * MPHIT blocks per Matriplex block, dense (hit block index block*nlayer + L).
+ * layer_mask[slot]: bit L set when layer L has a hit; absent layers are skipped
+ * (no propagation, no update); padding lanes carry mask 0 and are copied
+ * through unchanged.  semantics / params / chi2 as in p2r_forward (chi2 indexed
+ * by slot).  `tile_rounds` (1, 2 or 4; 0 = the tuned default) sets how many
+ * 256-track rounds a workgroup's tile holds: tracks of a tile are ordered by hit
+ * count before they are dealt to wavefronts, so larger tiles balance lanes
+ * better and re-read more hit lines.  The reference has no uneven batches
+ * (every event has ntrks tracks and every track nlayer hits, tbb.cpp:15-34);
+ * with all masks full and equal event sizes the result equals p2r_forward's. */
+int p2r_forward_ragged(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out, const int64_t* blk_offsets,
+                       int32_t nevts, const uint32_t* layer_mask, int32_t nlayer, int32_t bsize, int semantics,
+                       const p2r_params* params, float* chi2, int32_t tile_rounds, void* stream);
+
+/* Gradient precision of p2r_forward_adjoint.
+ * P2R_GRAD_F32:   every layer in FP32 (the arithmetic of params->precision).
+ * P2R_GRAD_MIXED: the state carried from layer to layer (and the checkpoints)
+ *                 in FP64, each layer's reverse sweep in FP32.
+ * P2R_GRAD_F64:   everything in FP64. */
+typedef enum p2r_grad_precision { P2R_GRAD_F32 = 0, P2R_GRAD_MIXED = 1, P2R_GRAD_F64 = 2 } p2r_grad_precision;
+
+/* Forward pass plus hand-written reverse-mode adjoint of the whole chained fit
+ * (P2R_CHAINED semantics, uniform batch as in p2r_forward).  Objective per
+ * track: the sum over layers of the local chi2 r^T S^-1 r of each update.
+ * Memory: the track state is checkpointed every `checkpoint_stride` layers
+ * (0 = 4) in context workspace; the reverse sweep recomputes each segment
+ * from its checkpoint instead of storing every intermediate.
+ * Outputs (device memory; any may be NULL except trk_out):
+ *   trk_out     final fitted tracks (same as p2r_forward, P2R_CHAINED)
+ *   chi2        float[ntrk]: total chi2 per track (flat track index)
+ *   grad_par    float, MP6F blocks: d chi2 / d (initial x, y, z, 1/pt, phi, theta)
+ *   grad_hits   float, one MP3F block per hit block (index block*nlayer + L):
+ *               d chi2 / d (hit x, y, z)
+ *   grad_kfact  float[ntrk]: d chi2 / d kfact (at the kfact/b_scale used)
+ *   grad_global double[4*nlayer + 1], summed over all tracks in FP64 in a fixed
+ *               order (bit-reproducible): for each layer L the derivative with
+ *               respect to a rigid offset of that layer's hits --
+ *               [4L+0..2] translation dx, dy, dz, [4L+3] rotation dphi about z
+ *               (x' = x cos dphi - y sin dphi, ...) -- then [4*nlayer] the
+ *               derivative with respect to b_scale (field = nominal * b_scale).
+ * The reference has no derivatives (this path is new). */
+int p2r_forward_adjoint(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* shape,
+                        const p2r_params* params, int grad_precision, int32_t checkpoint_stride, float* chi2,
+                        float* grad_par, float* grad_hits, float* grad_kfact, double* grad_global, void* stream);
+
 /* Block-level mirrors of the reference entry points, over nblk Matriplex
  * blocks.  Pointers address the field arrays of the first block; consecutive
  * blocks are `trk_stride` floats apart for track-side arrays and `hit_stride`

@@ -53,6 +53,9 @@ class Oracle:
         L.orc_summary.argtypes = [vp, i, i, i, f, f, f, vp, vp]
         L.orc_opcount.argtypes = [vp]
         L.orc_set_trig_mode.argtypes = [i]
+        L.orc_chi2_f64.argtypes = [vp, vp, i, vp, vp, i, d]
+        L.orc_chi2_f64.restype = d
+        L.orc_chi2_grad_dual.argtypes = [vp, vp, i, vp, i, d, vp, vp]
 
     # -- block-level mirrors of the reference entry points --
     def propagate_block(self, bs, kfact, inErr, inPar, inChg, msP):
@@ -108,6 +111,24 @@ class Oracle:
                                   semantics, kfact, nthreads, _ptr(m))
         assert rc == 0
         return par, cov, c2
+
+    # -- gradients of the chained fit's total chi2 (one track) --
+    def chi2_f64(self, par, cov, q, hitpos, hits_aos, nlayer, kfact=None):
+        kfact = float(self.kfact_tbb()) if kfact is None else kfact
+        return self.lib.orc_chi2_f64(_ptr(np.ascontiguousarray(par, np.float64)),
+                                     _ptr(np.ascontiguousarray(cov, np.float32)), int(q),
+                                     _ptr(np.ascontiguousarray(hitpos, np.float64)),
+                                     _ptr(np.ascontiguousarray(hits_aos, np.float32)), nlayer, kfact)
+
+    def chi2_grad_dual(self, par, cov, q, hits_aos, nlayer, kfact=None):
+        """FP64 forward-mode gradient: [d/dpar (6), d/dhit (3 per layer), d/dkfact], and the chi2."""
+        kfact = float(self.kfact_tbb()) if kfact is None else kfact
+        g = np.zeros(6 + 3 * nlayer + 1)
+        c2 = np.zeros(1)
+        self.lib.orc_chi2_grad_dual(_ptr(np.ascontiguousarray(par, np.float32)),
+                                    _ptr(np.ascontiguousarray(cov, np.float32)), int(q),
+                                    _ptr(np.ascontiguousarray(hits_aos, np.float32)), nlayer, kfact, _ptr(g), _ptr(c2))
+        return g, float(c2[0])
 
     def kfact_tbb(self):
         return self.lib.orc_kfact_tbb()

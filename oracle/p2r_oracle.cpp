@@ -340,6 +340,73 @@ void orc_summary(const float* out, int nevts, int ntrks, int bs, float hx_, floa
   counts[1] = nfail;
 }
 
+// ---- reference gradients of the chained fit's total chi2 --------------------
+// Sum over layers of the local chi2 (r^T S^-1 r, the quantity this project
+// adds to the reference's KalmanUpdate) of one track in chained semantics, as
+// a function of: the 6 initial parameters, the 3 coordinates of every hit
+// and kfact.  Input layout per track (AoS): par[6], cov[21], hits[nlayer][9]
+// (pos 3, cov 6).  Gradient layout: [0..5] d/dpar, [6 + 3*l + c] d/d hit l
+// coordinate c, [6 + 3*nlayer] d/dkfact.
+
+}  // extern "C"
+
+namespace {
+template <class T>
+T chained_chi2(const T par[6], const float cov[21], int q, const T* hitpos, const float* hits, int nlayer, T kf) {
+  Trk<T> cur;
+  for (int f = 0; f < 6; ++f) cur.par[f] = par[f];
+  for (int f = 0; f < 21; ++f) cur.cov[f] = T(cov[f]);
+  T tot = T(0.f);
+  for (int l = 0; l < nlayer; ++l) {
+    Hit<T> h;
+    for (int c = 0; c < 3; ++c) h.pos[c] = hitpos[3 * l + c];
+    for (int c = 0; c < 6; ++c) h.cov[c] = T(hits[9 * l + 3 + c]);
+    Trk<T> prop;
+    propagate_to_r(cur, q, h.pos, kf, prop);
+    T c2;
+    kalman_update(prop, h, &c2);
+    tot = tot + c2;
+    cur = prop;
+  }
+  return tot;
+}
+}  // namespace
+
+extern "C" {
+
+// FP64 value of the chained total chi2 with the inputs given in double (for
+// central finite differences).  par/hitpos are the (perturbed) variables.
+double orc_chi2_f64(const double par[6], const float cov[21], int q, const double* hitpos, const float* hits,
+                    int nlayer, double kfact) {
+  return chained_chi2<double>(par, cov, q, hitpos, hits, nlayer, kfact);
+}
+
+// Forward-mode (Dual, FP64) gradient, 16 directions per pass.
+void orc_chi2_grad_dual(const float par[6], const float cov[21], int q, const float* hits, int nlayer, double kfact,
+                        double* grad, double* chi2) {
+  constexpr int ND = 16;
+  using D = Dual<ND>;
+  const int nvar = 6 + 3 * nlayer + 1;
+  std::vector<D> hp(3 * nlayer);
+  for (int c0 = 0; c0 < nvar; c0 += ND) {
+    D p[6], kf(kfact);
+    for (int f = 0; f < 6; ++f) {
+      p[f] = D(par[f]);
+      if (f >= c0 && f < c0 + ND) p[f].d[f - c0] = 1.0;
+    }
+    for (int l = 0; l < nlayer; ++l)
+      for (int c = 0; c < 3; ++c) {
+        const int v = 6 + 3 * l + c;
+        hp[3 * l + c] = D(hits[9 * l + c]);
+        if (v >= c0 && v < c0 + ND) hp[3 * l + c].d[v - c0] = 1.0;
+      }
+    if (nvar - 1 >= c0 && nvar - 1 < c0 + ND) kf.d[nvar - 1 - c0] = 1.0;
+    const D tot = chained_chi2<D>(p, cov, q, hp.data(), hits, nlayer, kf);
+    for (int v = c0; v < c0 + ND && v < nvar; ++v) grad[v] = tot.d[v - c0];
+    if (chi2) *chi2 = tot.v;
+  }
+}
+
 // ---- op count ---------------------------------------------------------------
 // FLOPs of one propagateToR + KalmanUpdate (+ the chi2 this project adds) for
 // one lane.  out[0..5] = add, mul, div, sqrt, trig, trivial; out[6] = chi2 ops

@@ -14,6 +14,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <new>
 
 using namespace p2r;
@@ -92,6 +93,319 @@ __global__ __launch_bounds__(kThreads) void fwd_uniform_kernel(const float* __re
   if (CHI2) chi2[g] = c2;
 }
 
+
+// ---------------------------------------------------------------------------
+// Uneven batch: persistent workgroups walk tiles of kRagThreads*ROUNDS
+// consecutive slots.  Within a tile, slots are ordered by hit count (stable,
+// descending) and dealt to the workgroup's lanes round by round, so the 64
+// lanes of a wavefront carry tracks with (nearly) the same number of present
+// layers; each lane then iterates only its own present layers.  Tiles are
+// assigned statically (tile = blockIdx.x + i*gridDim.x): deterministic, no
+// device-side counters to reset.
+constexpr int kRagThreads = 256;
+constexpr int kRagWaves = kRagThreads / 64;
+
+template <int SEM, bool CHI2, bool EXACT>
+__device__ __forceinline__ void ragged_slot(const float* __restrict__ trk, const float* __restrict__ hits,
+                                            float* __restrict__ out, float* __restrict__ chi2, uint32_t slot,
+                                            uint32_t m, uint32_t bs, int nlayer, float kfact) {
+  const uint32_t blk = slot / bs, lane = slot - blk * bs;
+  const float* tb = trk + (size_t)blk * 28 * bs + lane;
+  Trk in;
+  load_trk(tb, bs, in);
+  const int q = reinterpret_cast<const int*>(tb)[(size_t)27 * bs];
+  const float k = (float)q * kfact;
+  const float* hb = hits + (size_t)blk * nlayer * 9 * bs + lane;
+  const size_t hstride = (size_t)9 * bs;
+  Trk cur = in;
+  float c2 = 0.f;
+  Hit hn;
+  if (m) load_hit(hb + (size_t)__builtin_ctz(m) * hstride, bs, hn);
+  while (m) {
+    m &= m - 1u;
+    const Hit h = hn;
+    if (m) load_hit(hb + (size_t)__builtin_ctz(m) * hstride, bs, hn);
+    Trk src;
+    if (SEM == P2R_REFERENCE) {
+      src = in;
+      opaque(src);
+    } else {
+      src = cur;
+    }
+    Trk prop;
+    Lane<EXACT>::propagate(src, k, h.pos[0], h.pos[1], prop);
+    float c = Lane<EXACT>::update(prop, h);
+    if (SEM == P2R_REFERENCE) {
+      opaque(prop);
+      opaque(c);
+    }
+    if (CHI2) c2 += c;
+    cur = prop;
+  }
+  float* ob = out + (size_t)blk * 28 * bs + lane;
+  store_trk(ob, bs, cur);
+  reinterpret_cast<int*>(ob)[(size_t)27 * bs] = q;
+  if (CHI2) chi2[slot] = c2;
+}
+
+template <int SEM, bool CHI2, bool EXACT, int ROUNDS>
+__global__ __launch_bounds__(kRagThreads) void fwd_ragged_kernel(const float* __restrict__ trk,
+                                                                 const float* __restrict__ hits,
+                                                                 float* __restrict__ out, float* __restrict__ chi2,
+                                                                 const uint32_t* __restrict__ mask, uint32_t nslot,
+                                                                 uint32_t bs, int nlayer, float kfact,
+                                                                 uint32_t ntiles) {
+  constexpr int kTile = kRagThreads * ROUNDS;
+  __shared__ uint32_t hist[kRagWaves * ROUNDS][33];  // per (wave, round): tracks with each hit count
+  __shared__ uint32_t start[33];                       // first rank of each hit count in the tile
+  __shared__ uint16_t order[kTile];                    // rank -> tile-local slot
+  __shared__ uint32_t smask[kTile];
+  const int tid = threadIdx.x, wave = tid >> 6, wl = tid & 63;
+  const uint64_t lt_mask = (wl == 0) ? 0ull : (~0ull >> (64 - wl));
+  for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint32_t base = tile * (uint32_t)kTile;
+    uint32_t key[ROUNDS], lrank[ROUNDS];
+    // bits at or above nlayer name no hit block: ignored
+    const uint32_t layer_bits = nlayer >= 32 ? ~0u : ((1u << nlayer) - 1u);
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+      const uint32_t s = base + r * kRagThreads + tid;
+      const uint32_t m = s < nslot ? (mask[s] & layer_bits) : 0u;
+      smask[r * kRagThreads + tid] = m;
+      key[r] = s < nslot ? (uint32_t)__builtin_popcount(m) : 33u;  // out-of-range slots sort last
+      for (uint32_t v = 0; v < 33; ++v) {
+        const uint64_t b = __ballot(key[r] == v);
+        if (key[r] == v) lrank[r] = (uint32_t)__popcll(b & lt_mask);
+        if (wl == 0) hist[r * kRagWaves + wave][v] = (uint32_t)__popcll(b);
+      }
+    }
+    __syncthreads();
+    if (tid < 33) {  // total per count, then exclusive prefix in descending count order
+      uint32_t t = 0;
+      for (int g = 0; g < kRagWaves * ROUNDS; ++g) t += hist[g][tid];
+      start[tid] = t;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t acc = 0;
+      for (int v = 32; v >= 0; --v) {
+        const uint32_t t = start[v];
+        start[v] = acc;
+        acc += t;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+      if (key[r] < 33) {
+        uint32_t rank = start[key[r]] + lrank[r];
+        const int g0 = r * kRagWaves + wave;
+        for (int g = 0; g < g0; ++g) rank += hist[g][key[r]];
+        order[rank] = (uint16_t)(r * kRagThreads + tid);
+      }
+    }
+    __syncthreads();
+    const uint32_t nvalid = nslot > base ? min((uint32_t)kTile, nslot - base) : 0u;
+#pragma unroll 1
+    for (int r = 0; r < ROUNDS; ++r) {
+      const uint32_t rank = r * kRagThreads + tid;
+      if (rank < nvalid) {
+        const uint32_t local = order[rank];
+        ragged_slot<SEM, CHI2, EXACT>(trk, hits, out, chi2, base + local, smask[local], bs, nlayer, kfact);
+      }
+    }
+    __syncthreads();  // order/smask/hist are rewritten by the next tile
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// Forward + adjoint of the chained fit.  One lane per track.  Forward pass in
+// the state precision RS, checkpoints every CS layers (checkpoint 0 is the
+// input track, not stored); reverse sweep per segment: the segment's layer
+// inputs are recomputed from its checkpoint (registers), then each layer's
+// values are recomputed and back-propagated in the layer precision.
+constexpr int kAdjThreads = 256;
+
+template <class R, bool EXACT> struct MathOf;
+template <> struct MathOf<float, true> {
+  using T = exact::Trk; using H = exact::Hit;
+  static __device__ __forceinline__ float fwd(const T& in, float k, const H& h, T& out) {
+    exact::LayerFwd f; exact::layer_forward_values(in, k, h, f, out); return exact::layer_chi2(f);
+  }
+  static __device__ __forceinline__ void adj(const T& in, float k, const H& h, T& bar, float& kb, float hb[3]) {
+    exact::layer_adjoint(in, k, h, bar, kb, hb);
+  }
+};
+template <> struct MathOf<float, false> {
+  using T = fast::Trk; using H = fast::Hit;
+  static __device__ __forceinline__ float fwd(const T& in, float k, const H& h, T& out) {
+    fast::LayerFwd f; fast::layer_forward_values(in, k, h, f, out); return fast::layer_chi2(f);
+  }
+  static __device__ __forceinline__ void adj(const T& in, float k, const H& h, T& bar, float& kb, float hb[3]) {
+    fast::layer_adjoint(in, k, h, bar, kb, hb);
+  }
+};
+template <bool EXACT> struct MathOf<double, EXACT> {
+  using T = dbl::Trk; using H = dbl::Hit;
+  static __device__ __forceinline__ double fwd(const T& in, double k, const H& h, T& out) {
+    dbl::LayerFwd f; dbl::layer_forward_values(in, k, h, f, out); return dbl::layer_chi2(f);
+  }
+  static __device__ __forceinline__ void adj(const T& in, double k, const H& h, T& bar, double& kb, double hb[3]) {
+    dbl::layer_adjoint(in, k, h, bar, kb, hb);
+  }
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <class R> __device__ __forceinline__ void load_hit_as(const float* hb, uint32_t bs, HitT<R>& h) {
+  Hit hf;
+  load_hit(hb, bs, hf);
+  convert(hf, h);
+}
+
+template <class RS, class RL, bool EXACT, int CS>
+__global__ __launch_bounds__(kAdjThreads) void adjoint_kernel(
+    const float* __restrict__ trk, const float* __restrict__ hits, float* __restrict__ out, float* __restrict__ chi2,
+    float* __restrict__ gpar, float* __restrict__ ghit, float* __restrict__ gk, double* __restrict__ partials,
+    RS* __restrict__ ckpt, uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
+  using MS = MathOf<RS, EXACT>;
+  using ML = MathOf<RL, EXACT>;
+  constexpr int kWaves = kAdjThreads / 64;
+  extern __shared__ double lds_global[];  // [kWaves][4*nlayer + 1] when partials != nullptr
+  const uint32_t g = blockIdx.x * kAdjThreads + threadIdx.x;
+  const bool active = g < ntrk;
+  const uint32_t gg = active ? g : ntrk - 1;  // inactive lanes recompute a valid track, write nothing
+  const uint32_t blk = gg / bs, lane = gg - blk * bs;
+  const float* tb = trk + (size_t)blk * 28 * bs + lane;
+  const float* hb = hits + (size_t)blk * nlayer * 9 * bs + lane;
+  const size_t hstride = (size_t)9 * bs;
+  const int q = reinterpret_cast<const int*>(tb)[(size_t)27 * bs];
+  const float k = (float)q * kf;
+  Trk in_f;
+  load_trk(tb, bs, in_f);
+  TrkT<RS> cur;
+  convert(in_f, cur);
+  // ---- forward pass with checkpoints ----
+  RS c2 = 0;
+  const int nseg = (nlayer + CS - 1) / CS;
+  for (int l = 0; l < nlayer; ++l) {
+    if (l % CS == 0 && l > 0) {
+      RS* cp = ckpt + ((size_t)(l / CS - 1) * 27) * ntrk + gg;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) cp[(size_t)f * ntrk] = cur.par[f];
+#pragma unroll
+      for (int f = 0; f < 21; ++f) cp[(size_t)(6 + f) * ntrk] = cur.cov[f];
+    }
+    HitT<RS> h;
+    load_hit_as(hb + l * hstride, bs, h);
+    TrkT<RS> nxt;
+    c2 += MS::fwd(cur, (RS)k, h, nxt);
+    cur = nxt;
+  }
+  if (active) {
+    float* ob = out + (size_t)blk * 28 * bs + lane;
+    Trk of;
+    convert(cur, of);
+    store_trk(ob, bs, of);
+    reinterpret_cast<int*>(ob)[(size_t)27 * bs] = q;
+    if (chi2) chi2[g] = (float)c2;
+  }
+  // ---- reverse sweep ----
+  const int nglob = 4 * nlayer + 1;
+  const int wave = threadIdx.x >> 6, wl = threadIdx.x & 63;
+  TrkT<RL> bar;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) bar.par[f] = 0;
+#pragma unroll
+  for (int f = 0; f < 21; ++f) bar.cov[f] = 0;
+  RL kbar = 0;
+  for (int s = nseg - 1; s >= 0; --s) {
+    TrkT<RS> st[CS];
+    if (s == 0) {
+      convert(in_f, st[0]);
+    } else {
+      const RS* cp = ckpt + ((size_t)(s - 1) * 27) * ntrk + gg;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) st[0].par[f] = cp[(size_t)f * ntrk];
+#pragma unroll
+      for (int f = 0; f < 21; ++f) st[0].cov[f] = cp[(size_t)(6 + f) * ntrk];
+    }
+    const int l0 = s * CS, len = min(CS, nlayer - l0);
+#pragma unroll
+    for (int j = 1; j < CS; ++j) {
+      if (j < len) {
+        HitT<RS> h;
+        load_hit_as(hb + (l0 + j - 1) * hstride, bs, h);
+        MS::fwd(st[j - 1], (RS)k, h, st[j]);
+      }
+    }
+#pragma unroll
+    for (int j = CS - 1; j >= 0; --j) {
+      if (j < len) {
+        const int l = l0 + j;
+        HitT<RL> h;
+        load_hit_as(hb + l * hstride, bs, h);
+        TrkT<RL> in_l;
+        convert(st[j], in_l);
+        RL hbar[3];
+        ML::adj(in_l, (RL)k, h, bar, kbar, hbar);
+        if (active && ghit) {
+          float* gh = ghit + ((size_t)blk * nlayer + l) * 3 * bs + lane;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) gh[(size_t)c * bs] = (float)hbar[c];
+        }
+        if (partials) {
+          const double hx = h.pos[0], hy = h.pos[1];
+          const double a = active ? (double)hbar[0] : 0.0, b = active ? (double)hbar[1] : 0.0,
+                       cz = active ? (double)hbar[2] : 0.0;
+          const double v0 = wave_sum(a), v1 = wave_sum(b), v2 = wave_sum(cz), v3 = wave_sum(-hy * a + hx * b);
+          if (wl == 0) {
+            lds_global[wave * nglob + 4 * l + 0] = v0;
+            lds_global[wave * nglob + 4 * l + 1] = v1;
+            lds_global[wave * nglob + 4 * l + 2] = v2;
+            lds_global[wave * nglob + 4 * l + 3] = v3;
+          }
+        }
+      }
+    }
+  }
+  if (active) {
+    if (gpar) {
+      float* gp = gpar + (size_t)blk * 6 * bs + lane;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) gp[(size_t)f * bs] = (float)bar.par[f];
+    }
+    if (gk) gk[g] = (float)((RL)q * kbar);  // d/dkfact = q d/dk
+  }
+  if (partials) {
+    // b_scale: kfact_eff = kfact / b  =>  d/db = -kfact_eff * d/dkfact_eff at b = 1 scaled by 1/b; the
+    // host passes kf = kfact/b, so d/db = -(kf / b) q kbar; written here for b folded into kf (b = 1 form)
+    const double vb = wave_sum(active ? -(double)kf * (double)q * (double)kbar : 0.0);
+    if (wl == 0) lds_global[wave * nglob + 4 * nlayer] = vb;
+    __syncthreads();
+    for (int j = threadIdx.x; j < nglob; j += kAdjThreads) {
+      double acc = 0.0;
+      for (int w = 0; w < kWaves; ++w) acc += lds_global[w * nglob + j];
+      partials[(size_t)blockIdx.x * nglob + j] = acc;
+    }
+  }
+}
+
+// grad_global[j] = sum over workgroups of partials[b][j], in workgroup order.
+__global__ void finalize_global_kernel(const double* __restrict__ partials, double* __restrict__ out, int nblocks,
+                                       int nglob, double bscale_factor) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nglob) return;
+  double acc = 0.0;
+  for (int b = 0; b < nblocks; ++b) acc += partials[(size_t)b * nglob + j];
+  out[j] = (j == nglob - 1) ? acc * bscale_factor : acc;
+}
+
 // Block-level propagateToR over nblk blocks with caller strides.
 __global__ __launch_bounds__(kThreads) void propagate_blocks_kernel(const float* __restrict__ inErr,
                                                                     const float* __restrict__ inPar,
@@ -147,6 +461,7 @@ __global__ __launch_bounds__(kThreads) void kalman_blocks_kernel(float* __restri
 }
 
 inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+inline unsigned grid_for_n(uint64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
 template <int SEM, bool CHI2>
 void launch_uniform_p(bool exact, dim3 grid, dim3 block, hipStream_t st, const float* T, const float* H, float* O,
@@ -165,6 +480,62 @@ void launch_uniform(int sem, bool chi2_on, bool exact, dim3 grid, dim3 block, hi
   }
 }
 
+
+template <int SEM, bool CHI2, bool EXACT, int ROUNDS>
+int launch_ragged_t(p2r_ctx* c, hipStream_t st, const float* T, const float* H, float* O, float* chi2,
+                    const uint32_t* M, uint32_t nslot, uint32_t bs, int nlayer, float kf) {
+  constexpr int kTile = kRagThreads * ROUNDS;
+  const uint32_t ntiles = (nslot + kTile - 1) / kTile;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fwd_ragged_kernel<SEM, CHI2, EXACT, ROUNDS>,
+                                                   kRagThreads, 0) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  const uint32_t grid = std::min<uint32_t>(ntiles, (uint32_t)(c->num_cus * per_cu));
+  fwd_ragged_kernel<SEM, CHI2, EXACT, ROUNDS><<<grid, kRagThreads, 0, st>>>(T, H, O, chi2, M, nslot, bs, nlayer, kf,
+                                                                           ntiles);
+  return P2R_OK;
+}
+template <int SEM, bool CHI2, bool EXACT>
+int launch_ragged_r(int rounds, p2r_ctx* c, hipStream_t st, const float* T, const float* H, float* O, float* chi2,
+                    const uint32_t* M, uint32_t nslot, uint32_t bs, int nlayer, float kf) {
+  switch (rounds) {
+    case 1: return launch_ragged_t<SEM, CHI2, EXACT, 1>(c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
+    case 2: return launch_ragged_t<SEM, CHI2, EXACT, 2>(c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
+    default: return launch_ragged_t<SEM, CHI2, EXACT, 4>(c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
+  }
+}
+template <int SEM>
+int launch_ragged_s(bool chi2_on, bool exact, int rounds, p2r_ctx* c, hipStream_t st, const float* T, const float* H,
+                    float* O, float* chi2, const uint32_t* M, uint32_t nslot, uint32_t bs, int nlayer, float kf) {
+  if (chi2_on) {
+    if (exact) return launch_ragged_r<SEM, true, true>(rounds, c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
+    return launch_ragged_r<SEM, true, false>(rounds, c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
+  }
+  if (exact) return launch_ragged_r<SEM, false, true>(rounds, c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
+  return launch_ragged_r<SEM, false, false>(rounds, c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
+}
+
+template <class RS, class RL, bool EXACT, int CS>
+void launch_adjoint_t(unsigned grid, size_t lds, hipStream_t st, const float* T, const float* H, float* O, float* chi2,
+                      float* gpar, float* ghit, float* gk, double* partials, void* ckpt, uint32_t ntrk, uint32_t bs,
+                      int nlayer, float kf) {
+  adjoint_kernel<RS, RL, EXACT, CS><<<grid, kAdjThreads, lds, st>>>(T, H, O, chi2, gpar, ghit, gk, partials,
+                                                                    static_cast<RS*>(ckpt), ntrk, bs, nlayer, kf);
+}
+template <bool EXACT, int CS>
+void launch_adjoint_p(int gp, unsigned grid, size_t lds, hipStream_t st, const float* T, const float* H, float* O,
+                      float* chi2, float* gpar, float* ghit, float* gk, double* partials, void* ckpt, uint32_t ntrk,
+                      uint32_t bs, int nlayer, float kf) {
+  if (gp == P2R_GRAD_F32)
+    launch_adjoint_t<float, float, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt, ntrk, bs,
+                                              nlayer, kf);
+  else if (gp == P2R_GRAD_MIXED)
+    launch_adjoint_t<double, float, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt, ntrk,
+                                               bs, nlayer, kf);
+  else
+    launch_adjoint_t<double, double, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt, ntrk,
+                                                bs, nlayer, kf);
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -249,6 +620,100 @@ int p2r_forward(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out,
   launch_uniform(semantics, chi2 != nullptr, p.precision == P2R_EXACT, grid, block, st, T, H, O, chi2, n, bs,
                  s->nlayer, kf);
   return c->check_launch("p2r_forward");
+}
+
+
+int p2r_forward_ragged(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out, const int64_t* blk_offsets,
+                       int32_t nevts, const uint32_t* layer_mask, int32_t nlayer, int32_t bsize, int semantics,
+                       const p2r_params* params, float* chi2, int32_t tile_rounds, void* stream) {
+  if (!c) return P2R_ERR_ARG;
+  if (!trk_in || !hits || !trk_out || !blk_offsets || !layer_mask) return c->fail(P2R_ERR_ARG, "null pointer argument");
+  if (semantics != P2R_REFERENCE && semantics != P2R_CHAINED) return c->fail(P2R_ERR_ARG, "bad semantics %d", semantics);
+  if (nevts < 0 || nlayer < 1 || nlayer > 32 || bsize < 1 || bsize > 1024)
+    return c->fail(P2R_ERR_SHAPE, "bad shape nevts=%d nlayer=%d bsize=%d (nlayer <= 32 for the 32-bit mask)", nevts,
+                   nlayer, bsize);
+  if (tile_rounds != 0 && tile_rounds != 1 && tile_rounds != 2 && tile_rounds != 4)
+    return c->fail(P2R_ERR_ARG, "tile_rounds must be 0, 1, 2 or 4");
+  if (blk_offsets[0] != 0) return c->fail(P2R_ERR_SHAPE, "blk_offsets[0] must be 0");
+  for (int32_t e = 0; e < nevts; ++e)
+    if (blk_offsets[e + 1] < blk_offsets[e]) return c->fail(P2R_ERR_SHAPE, "blk_offsets decrease at event %d", e);
+  const uint64_t nslot = (uint64_t)blk_offsets[nevts] * (uint64_t)bsize;
+  if (nslot >= (1ull << 31)) return c->fail(P2R_ERR_SHAPE, "too many slots (%llu)", (unsigned long long)nslot);
+  const p2r_params p = params ? *params : p2r_default_params();
+  if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
+  if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
+  if (nslot == 0) return P2R_OK;
+  const int rounds = tile_rounds ? tile_rounds : 2;
+  DeviceGuard g(c->device);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  auto T = static_cast<const float*>(trk_in);
+  auto H = static_cast<const float*>(hits);
+  auto O = static_cast<float*>(trk_out);
+  const float kf = p.kfact / p.b_scale;
+  const bool exact = p.precision == P2R_EXACT;
+  if (semantics == P2R_REFERENCE)
+    launch_ragged_s<P2R_REFERENCE>(chi2 != nullptr, exact, rounds, c, st, T, H, O, chi2, layer_mask, (uint32_t)nslot,
+                                   (uint32_t)bsize, nlayer, kf);
+  else
+    launch_ragged_s<P2R_CHAINED>(chi2 != nullptr, exact, rounds, c, st, T, H, O, chi2, layer_mask, (uint32_t)nslot,
+                                 (uint32_t)bsize, nlayer, kf);
+  return c->check_launch("p2r_forward_ragged");
+}
+
+
+int p2r_forward_adjoint(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* s,
+                        const p2r_params* params, int grad_precision, int32_t checkpoint_stride, float* chi2,
+                        float* grad_par, float* grad_hits, float* grad_kfact, double* grad_global, void* stream) {
+  if (!c) return P2R_ERR_ARG;
+  if (!trk_in || !hits || !trk_out || !s) return c->fail(P2R_ERR_ARG, "null pointer argument");
+  if (s->nevts < 0 || s->ntrks < 0 || s->nlayer < 1 || s->nlayer > 64 || s->bsize < 1 || s->bsize > 1024)
+    return c->fail(P2R_ERR_SHAPE, "bad shape nevts=%d ntrks=%d nlayer=%d bsize=%d", s->nevts, s->ntrks, s->nlayer,
+                   s->bsize);
+  if (s->ntrks % s->bsize != 0) return c->fail(P2R_ERR_SHAPE, "ntrks must be a multiple of bsize");
+  if (grad_precision < P2R_GRAD_F32 || grad_precision > P2R_GRAD_F64)
+    return c->fail(P2R_ERR_ARG, "bad grad_precision %d", grad_precision);
+  const int cs = checkpoint_stride ? checkpoint_stride : 4;
+  if (cs != 2 && cs != 4 && cs != 8) return c->fail(P2R_ERR_ARG, "checkpoint_stride must be 0, 2, 4 or 8");
+  const uint64_t ntrk = (uint64_t)s->nevts * (uint64_t)s->ntrks;
+  if (ntrk >= (1ull << 31)) return c->fail(P2R_ERR_SHAPE, "too many tracks");
+  const p2r_params p = params ? *params : p2r_default_params();
+  if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
+  if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
+  if (ntrk == 0) return P2R_OK;
+  DeviceGuard g(c->device);
+  const int nlayer = s->nlayer, nglob = 4 * nlayer + 1;
+  const unsigned grid = grid_for_n(ntrk, kAdjThreads);
+  const size_t sz = grad_precision == P2R_GRAD_F32 ? sizeof(float) : sizeof(double);
+  const size_t nck = (size_t)((nlayer + cs - 1) / cs - 1);
+  const size_t ck_bytes = std::max<size_t>(nck, 1) * 27 * ntrk * sz;
+  const size_t part_bytes = grad_global ? (size_t)grid * nglob * sizeof(double) : 0;
+  char* ws = static_cast<char*>(c->workspace(ck_bytes + part_bytes + 256));
+  if (!ws) return c->fail(P2R_ERR_NOMEM, "workspace of %zu bytes", ck_bytes + part_bytes);
+  double* partials = grad_global ? reinterpret_cast<double*>(ws + ((ck_bytes + 255) & ~(size_t)255)) : nullptr;
+  const size_t lds = grad_global ? (size_t)(kAdjThreads / 64) * nglob * sizeof(double) : 0;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const float kf = p.kfact / p.b_scale;
+  auto T = static_cast<const float*>(trk_in);
+  auto H = static_cast<const float*>(hits);
+  auto O = static_cast<float*>(trk_out);
+  const uint32_t n = (uint32_t)ntrk, bs = (uint32_t)s->bsize;
+  const bool exact = p.precision == P2R_EXACT;
+#define P2R_ADJ(E, CS)                                                                                           \
+  launch_adjoint_p<E, CS>(grad_precision, grid, lds, st, T, H, O, chi2, grad_par, grad_hits, grad_kfact, partials, ws, \
+                          n, bs, nlayer, kf)
+  if (exact) {
+    if (cs == 2) P2R_ADJ(true, 2); else if (cs == 4) P2R_ADJ(true, 4); else P2R_ADJ(true, 8);
+  } else {
+    if (cs == 2) P2R_ADJ(false, 2); else if (cs == 4) P2R_ADJ(false, 4); else P2R_ADJ(false, 8);
+  }
+#undef P2R_ADJ
+  int rc = c->check_launch("p2r_forward_adjoint");
+  if (rc != P2R_OK || !grad_global) return rc;
+  // d/d b_scale = (d/d kfact_eff) * (-kfact / b^2); the kernel summed -kf q dchi2/dk = -kf dchi2/dkfact_eff,
+  // which equals b * d/db: divide by b.
+  finalize_global_kernel<<<(nglob + 63) / 64, 64, 0, st>>>(partials, grad_global, (int)grid, nglob,
+                                                           1.0 / (double)p.b_scale);
+  return c->check_launch("p2r_forward_adjoint (reduction)");
 }
 
 int p2r_propagateToR(p2r_ctx* c, const float* inErr, const float* inPar, const int32_t* inChg, const float* msP,

@@ -21,15 +21,16 @@ namespace p2r {
 
 constexpr int kNiter = 5;  // fixed-point steps of propagateToR, tbb.cpp:655
 
-struct Trk {
-  float par[6];   // x, y, z, 1/pt, phi, theta
-  float cov[21];  // packed lower-triangular symmetric
+template <class R> struct TrkT {
+  R par[6];   // x, y, z, 1/pt, phi, theta
+  R cov[21];  // packed lower-triangular symmetric
 };
-
-struct Hit {
-  float pos[3];
-  float cov[6];  // xx, yx, yy, zx, zy, zz
+template <class R> struct HitT {
+  R pos[3];
+  R cov[6];  // xx, yx, yy, zx, zy, zz
 };
+using Trk = TrkT<float>;
+using Hit = HitT<float>;
 
 __host__ __device__ constexpr int sym(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
 
@@ -46,27 +47,69 @@ __device__ __forceinline__ void opaque(Trk& t) {
 
 // Non-trivial entries of the propagation Jacobian (errorProp, tbb.cpp:753-807).
 // Rows 3 and 5 are unit rows; J22 = 1; J02 = J05 = J12 = J15 = J42 = J45 = 0.
-struct Jac {
-  float j00, j01, j03, j04;
-  float j10, j11, j13, j14;
-  float j20, j21, j23, j24, j25;
-  float j40, j41, j43, j44;
+template <class R> struct JacT {
+  R j00, j01, j03, j04;
+  R j10, j11, j13, j14;
+  R j20, j21, j23, j24, j25;
+  R j40, j41, j43, j44;
 };
 
-// Two instances of the per-lane math (see p2r_lane_math.inc).
+template <class A, class B> __device__ __forceinline__ void convert(const TrkT<A>& a, TrkT<B>& b) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) b.par[i] = (B)a.par[i];
+#pragma unroll
+  for (int i = 0; i < 21; ++i) b.cov[i] = (B)a.cov[i];
+}
+template <class A, class B> __device__ __forceinline__ void convert(const HitT<A>& a, HitT<B>& b) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) b.pos[i] = (B)a.pos[i];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) b.cov[i] = (B)a.cov[i];
+}
+
+// Instances of the per-lane math (see p2r_lane_math.inc) and of its adjoint
+// (p2r_adjoint.inc).
 namespace exact {
 #pragma clang fp contract(off)
+using real = float;
+using Trk = TrkT<real>;
+using Hit = HitT<real>;
+using Jac = JacT<real>;
 #define P2R_EXACT 1
+#define P2R_DOUBLE 0
 #include "p2r_lane_math.inc"
+#include "p2r_adjoint.inc"
 #undef P2R_EXACT
+#undef P2R_DOUBLE
 }  // namespace exact
 
 namespace fast {
 #pragma clang fp contract(fast)
+using real = float;
+using Trk = TrkT<real>;
+using Hit = HitT<real>;
+using Jac = JacT<real>;
 #define P2R_EXACT 0
+#define P2R_DOUBLE 0
 #include "p2r_lane_math.inc"
+#include "p2r_adjoint.inc"
 #undef P2R_EXACT
+#undef P2R_DOUBLE
 }  // namespace fast
+
+namespace dbl {
+#pragma clang fp contract(fast)
+using real = double;
+using Trk = TrkT<real>;
+using Hit = HitT<real>;
+using Jac = JacT<real>;
+#define P2R_EXACT 0
+#define P2R_DOUBLE 1
+#include "p2r_lane_math.inc"
+#include "p2r_adjoint.inc"
+#undef P2R_EXACT
+#undef P2R_DOUBLE
+}  // namespace dbl
 
 // Compile-time selection of the instance.
 template <bool EXACT> struct Lane;

@@ -77,6 +77,18 @@ class Context:
                                _stream(stream))
         self._check(st, "p2r_forward")
 
+    def forward_ragged(self, trk, hits, out, blk_offsets, layer_mask, nlayer, bsize, semantics="chained",
+                       params=None, chi2=None, stream=None, precision=None, tile_rounds=0):
+        """Uneven batch, one persistent launch (include/p2r.h p2r_forward_ragged).  blk_offsets: host int64
+        array of nevts+1 block offsets; trk/hits/out/layer_mask/chi2: device tensors."""
+        sem = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
+        p = params if params is not None else default_params(precision or "exact")
+        offs = np.ascontiguousarray(blk_offsets, np.int64)
+        st = lib().p2r_forward_ragged(self._h, _ptr(trk), _ptr(hits), _ptr(out), _ptr(offs), len(offs) - 1,
+                                      _ptr(layer_mask), int(nlayer), int(bsize), sem, C.byref(p), _ptr(chi2),
+                                      int(tile_rounds), _stream(stream))
+        self._check(st, "p2r_forward_ragged")
+
     def propagateToR(self, inErr, inPar, inChg, msP, outErr, outPar, bsize, nblk, trk_stride, hit_stride,
                      kfact=None, stream=None):
         """Mirror of propagateToR (tbb.cpp:656) over nblk blocks; pointers/tensors address field arrays."""

@@ -45,6 +45,8 @@ def lib():
             "p2r_error_string": ([C.c_int], C.c_char_p),
             "p2r_last_error": ([vp], C.c_char_p),
             "p2r_forward": ([vp, vp, vp, vp, C.POINTER(p2r_shape), C.c_int, C.POINTER(p2r_params), vp, vp], C.c_int),
+            "p2r_forward_ragged": ([vp, vp, vp, vp, vp, i32, vp, i32, i32, C.c_int, C.POINTER(p2r_params), vp, i32,
+                                    vp], C.c_int),
             "p2r_propagateToR": ([vp, vp, vp, vp, vp, vp, vp, i32, i64, i64, i64, f32, vp], C.c_int),
             "p2r_KalmanUpdate": ([vp, vp, vp, vp, vp, i32, i64, i64, i64, vp], C.c_int),
             "p2r_gen_reference": ([vp, vp, C.POINTER(p2r_shape)], C.c_int),

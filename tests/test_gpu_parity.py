@@ -229,3 +229,85 @@ def test_work_scales_with_layers(ctx, precision):
         print(f"work scaling {precision} {sem}: t10={times[10]:.4f} ms t20={times[20]:.4f} ms ratio={ratio:.3f}",
               flush=True)
         assert 1.6 < ratio < 2.4, (sem, times)
+
+
+# ---- uneven batch (persistent ragged kernel) --------------------------------
+
+def _ragged_inputs(nevts=40, median=600, bsize=32, nlayer=20, seed=11):
+    import p2r_amd
+    counts = p2r_amd.lognormal_counts(nevts, median=median, sigma=0.5, lo=30, hi=3000, seed=seed)
+    trk, hits, mask, offs = p2r_amd.gen_ragged(counts, nlayer, bsize, kmin=min(5, nlayer), kmax=min(20, nlayer))
+    return counts, trk, hits, mask, offs
+
+
+def run_ragged(ctx, trk, hits, mask, offs, nlayer, bsize, semantics, precision="exact", rounds=0):
+    dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
+    dm = torch.from_numpy(mask.view(np.int32)).cuda()
+    do = torch.empty_like(dt)
+    dc = torch.full((mask.size,), float("nan"), dtype=torch.float32, device="cuda")
+    ctx.forward_ragged(dt, dh, do, offs, dm, nlayer, bsize, semantics, chi2=dc, precision=precision,
+                       tile_rounds=rounds)
+    torch.cuda.synchronize()
+    return do.cpu().numpy(), dc.cpu().numpy()
+
+
+@pytest.mark.parametrize("rounds", [1, 2, 4])
+@pytest.mark.parametrize("semantics", ["chained", "reference"])
+def test_exact_ragged_matches_oracle(orc, ctx, semantics, rounds):
+    """Uneven events + masked layers, one persistent launch: bit-identical to the oracle's masked run."""
+    counts, trk, hits, mask, offs = _ragged_inputs()
+    nblk = int(offs[-1])
+    sem = 0 if semantics == "reference" else 1
+    gpu, c2g = run_ragged(ctx, trk, hits, mask, offs, 20, 32, semantics, rounds=rounds)
+    cpu, c2c = orc.run_f32(trk, hits, nblk, 20, 32, semantics=sem, chi2=True, mask=mask, cr_trig=True)
+    n = bit_mismatches(gpu.reshape(-1, 28, 32)[:, :27], cpu.reshape(-1, 28, 32)[:, :27])
+    nq = int((gpu.reshape(-1, 28, 32)[:, 27].view(np.int32) != cpu.reshape(-1, 28, 32)[:, 27].view(np.int32)).sum())
+    nc = bit_mismatches(c2g, c2c)
+    print(f"ragged {semantics} rounds={rounds}: {int(counts.sum())} tracks in {nblk} blocks, mismatches "
+          f"state={n} q={nq} chi2={nc}", flush=True)
+    assert n == 0 and nq == 0 and nc == 0
+
+
+def test_ragged_full_masks_equal_uniform(ctx):
+    """All layers present and equal event sizes: the ragged launch reproduces p2r_forward bit for bit."""
+    import p2r_amd
+    shape = Shape(6, 640, 20, 32)
+    trk, hits = p2r_amd.gen_uniform(shape)
+    offs = np.arange(shape.nevts + 1, dtype=np.int64) * shape.nb
+    mask = np.full(shape.ntrk_total, (1 << 20) - 1, np.uint32)
+    for sem in ("reference", "chained"):
+        uni, _ = run_gpu(ctx, trk, hits, shape, sem)
+        rag, _ = run_ragged(ctx, trk, hits, mask, offs, 20, 32, sem)
+        assert bit_mismatches(uni, rag) == 0
+
+
+def test_ragged_ignores_mask_bits_beyond_nlayer(orc, ctx):
+    counts, trk, hits, mask, offs = _ragged_inputs(nevts=8, nlayer=12)
+    junk = mask | np.uint32(0xFFFFF000)  # bits 12..31 name no hit block
+    gpu, _ = run_ragged(ctx, trk, hits, junk, offs, 12, 32, "chained")
+    cpu = orc.run_f32(trk, hits, int(offs[-1]), 12, 32, semantics=1, mask=mask, cr_trig=True)
+    assert bit_mismatches(gpu.reshape(-1, 28, 32)[:, :27], cpu.reshape(-1, 28, 32)[:, :27]) == 0
+
+
+def test_ragged_graph_replays_match_eager(ctx):
+    """The persistent launch captured in a HIP graph: every replay (with eager work in between) equals the
+    eager result bit for bit."""
+    counts, trk, hits, mask, offs = _ragged_inputs(nevts=16)
+    eager, _ = run_ragged(ctx, trk, hits, mask, offs, 20, 32, "chained")
+    dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
+    dm = torch.from_numpy(mask.view(np.int32)).cuda()
+    do = torch.zeros_like(dt)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        ctx.forward_ragged(dt, dh, do, offs, dm, 20, 32, "chained", stream=s)  # warm-up outside capture
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            ctx.forward_ragged(dt, dh, do, offs, dm, 20, 32, "chained", stream=s)
+    s.synchronize()
+    for i in range(3):
+        do.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert bit_mismatches(do.cpu().numpy(), eager) == 0, f"replay {i}"
+        _ = (dt * 2).sum().item()  # unrelated eager work between replays
