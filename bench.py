@@ -53,6 +53,9 @@ def main():
     ap.add_argument("--no-variants", action="store_true", help="time only the headline variant")
     ap.add_argument("--no-ragged", action="store_true", help="skip the uneven-batch (config 3) measurement")
     ap.add_argument("--ragged-events", type=int, default=500)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the fwd+adjoint (config 4) measurement")
+    ap.add_argument("--no-alignment", action="store_true", help="skip the alignment (config 5) measurement")
+    ap.add_argument("--alignment-tracks", type=int, default=1000000)
     ap.add_argument("--no-parity", action="store_true", help="skip the parity spot check")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (steps times) and exit: the "
@@ -91,6 +94,7 @@ def main():
     d_out = torch.empty_like(d_trk)
     ctx = p2r_amd.Context(0)
     stream = torch.cuda.current_stream()
+    args._shape, args._bufs = shape, (d_trk, d_hit, d_out)
 
     def step(sem=args.semantics, prec=args.precision):
         ctx.forward(d_trk, d_hit, d_out, shape, sem, stream=stream, precision=prec)
@@ -158,12 +162,23 @@ def main():
     except Exception as ex:  # the bench line is still printed; parity is covered by the tests
         parity_res = {"error": repr(ex)}
 
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_bench(ctx, args,
+                                {(v["semantics"], v["precision"]): v["ms_per_step"] for v in variants},
+                                (args.semantics, args.precision, ms))
     ragged = None
     if not args.no_ragged:
         del d_trk, d_hit, d_out
+        args._bufs = None
         torch.cuda.empty_cache()
         ragged = ragged_bench(ctx, args, {v["precision"]: v for v in variants if v["semantics"] == "chained"},
                               (args.semantics, args.precision, value))
+
+    alignment = None
+    if not args.no_alignment:
+        torch.cuda.empty_cache()
+        alignment = alignment_bench(ctx, args)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -197,12 +212,98 @@ def main():
                      "kernel": "fwd_uniform_kernel"},
         "cpu_baseline": cpu,
         "variants": variants,
+        "adjoint": adjoint,
         "ragged": ragged,
+        "alignment": alignment,
         "parity": parity_res,
         "wall_s_timed": wall,
         "gen_s": gen_s,
     }
     print(json.dumps(line), flush=True)
+
+
+def _timed(fn, reps=5, warm=2):
+    import torch
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def adjoint_bench(ctx, args, variant_ms, headline):
+    """the headline shape, chained fit + reverse-mode gradient of the total chi2 with
+    respect to the initial parameters, every hit coordinate and kfact (checkpoints every 4 layers).
+    Reported as the time of one fused forward+adjoint launch and its ratio to the plain chained forward."""
+    import torch
+    shape = args._shape
+    d_trk, d_hit, d_out = args._bufs
+    n = shape.ntrk_total
+    c2 = torch.empty(n, device="cuda")
+    gpar = torch.empty(shape.nblk * 6 * shape.bsize, device="cuda")
+    ghit = torch.empty(shape.nblk * shape.nlayer * 3 * shape.bsize, device="cuda")
+    gk = torch.empty(n, device="cuda")
+    res = {"shape": list(shape.tuple()), "checkpoint_stride": 4}
+    for prec in ("exact", "fast"):
+        fwd_ms = variant_ms.get(("chained", prec))
+        if fwd_ms is None and headline[0] == "chained" and headline[1] == prec:
+            fwd_ms = headline[2]
+        for gp in ("f32", "mixed", "f64"):
+            ms = _timed(lambda: ctx.forward_adjoint(d_trk, d_hit, d_out, shape, grad_precision=gp, precision=prec,
+                                                    chi2=c2, grad_par=gpar, grad_hits=ghit, grad_kfact=gk))
+            res[f"{prec}_{gp}"] = {"ms": ms, "updates_per_s": shape.ntrk_total * shape.nlayer / (ms * 1e-3),
+                                   "ratio_to_chained_forward": (ms / fwd_ms) if fwd_ms else None}
+    return res
+
+
+def alignment_bench(ctx, args):
+    """1e6 tracks x 40 layers, hits generated with hidden per-layer rigid offsets
+    (dx, dy, dz ~ N(0, 50 um), dphi ~ N(0, 1e-4)) and field scale 1 + N(0, 1e-3); the fit runs at nominal
+    geometry and field; fwd+adjoint gives d(sum chi2)/d(160 offsets + field scale), reduced in FP64.
+    Reports the time and the relative error of the FP32 / mixed gradients against the FP64 one."""
+    import numpy as np
+    import torch
+
+    import p2r_amd
+    from p2r_amd import Shape
+    nl, bs = 40, args.bsize
+    ntrks = args.alignment_tracks
+    shape = Shape(1, ntrks - ntrks % bs, nl, bs)
+    rng = np.random.default_rng(20240905)
+    offsets = np.zeros((nl, 4), np.float32)
+    offsets[:, :3] = rng.normal(0, 50e-4, (nl, 3))   # 50 um in cm
+    offsets[:, 3] = rng.normal(0, 1e-4, nl)
+    bscale_truth = float(1 + rng.normal(0, 1e-3))
+    t0 = time.time()
+    cfg = p2r_amd.gen_config(r0=20.0, dr=2.5, layer_offsets=offsets.ravel(), b_scale_truth=bscale_truth)
+    trk, hits = p2r_amd.gen_uniform(shape, cfg)
+    gen_s = time.time() - t0
+    dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
+    del trk, hits
+    do = torch.empty_like(dt)
+    res = {"tracks": shape.ntrk_total, "nlayer": nl, "gen_s": gen_s, "b_scale_truth": bscale_truth}
+    grads = {}
+    for gp in ("f64", "mixed", "f32"):
+        gg = torch.empty(4 * nl + 1, dtype=torch.float64, device="cuda")
+        ms = _timed(lambda: ctx.forward_adjoint(dt, dh, do, shape, grad_precision=gp, precision="exact",
+                                                grad_global=gg), reps=3, warm=1)
+        grads[gp] = gg.cpu().numpy()
+        res[gp] = {"ms": ms, "updates_per_s": shape.ntrk_total * nl / (ms * 1e-3)}
+    ref = grads["f64"]
+    for gp in ("mixed", "f32"):
+        res[gp]["rel_err_vs_f64_norm"] = float(np.linalg.norm(grads[gp] - ref) / np.linalg.norm(ref))
+        res[gp]["rel_err_vs_f64_max_component"] = float(np.max(np.abs(grads[gp] - ref) / (np.abs(ref) + 1e-30)))
+    g = ref[:4 * nl].reshape(nl, 4)
+    # the gradient at nominal points along the hidden misalignment: correlation of -grad with the truth
+    res["corr_minus_grad_vs_truth"] = {k: float(np.corrcoef(-g[:, i], offsets[:, i])[0, 1])
+                                       for i, k in enumerate(("dx", "dy", "dz", "dphi"))}
+    res["d_chi2_d_bscale"] = float(ref[-1])
+    return res
 
 
 def ragged_bench(ctx, args, uniform_chained, headline):

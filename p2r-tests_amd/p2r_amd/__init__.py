@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._capi import (P2R_CHAINED, P2R_EXACT, P2R_FAST, P2R_ERR_ARG, P2R_ERR_HIP, P2R_ERR_NOMEM, P2R_ERR_SHAPE, P2R_ERR_UNSUPPORTED,
+from ._capi import (P2R_CHAINED, P2R_EXACT, P2R_FAST, P2R_GRAD_F32, P2R_GRAD_MIXED, P2R_GRAD_F64, P2R_ERR_ARG, P2R_ERR_HIP, P2R_ERR_NOMEM, P2R_ERR_SHAPE, P2R_ERR_UNSUPPORTED,
                     P2R_OK, P2R_REFERENCE, P2RError, lib, p2r_gen_config, p2r_params, p2r_shape)
 from .matriplex import Shape, tracks_to_arrays, hits_block_view  # noqa: F401
 
@@ -34,6 +34,7 @@ def _stream(stream):
 
 
 PRECISION = {"exact": P2R_EXACT, "fast": P2R_FAST}
+GRAD_PRECISION = {"f32": P2R_GRAD_F32, "mixed": P2R_GRAD_MIXED, "f64": P2R_GRAD_F64}
 
 
 def default_params(precision="exact", b_scale=1.0):
@@ -88,6 +89,20 @@ class Context:
                                       _ptr(layer_mask), int(nlayer), int(bsize), sem, C.byref(p), _ptr(chi2),
                                       int(tile_rounds), _stream(stream))
         self._check(st, "p2r_forward_ragged")
+
+    def forward_adjoint(self, trk, hits, out, shape, params=None, grad_precision="f32", checkpoint_stride=0,
+                        chi2=None, grad_par=None, grad_hits=None, grad_kfact=None, grad_global=None, stream=None,
+                        precision=None):
+        """Chained fit plus reverse-mode gradient of the total chi2 (include/p2r.h p2r_forward_adjoint).
+        grad_par: MP6F blocks; grad_hits: MP3F per hit block; grad_kfact: per track; grad_global: float64
+        [4*nlayer+1] (per-layer rigid offsets dx, dy, dz, dphi, then the field scale)."""
+        sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
+        p = params if params is not None else default_params(precision or "exact")
+        gp = GRAD_PRECISION[grad_precision] if isinstance(grad_precision, str) else int(grad_precision)
+        st = lib().p2r_forward_adjoint(self._h, _ptr(trk), _ptr(hits), _ptr(out), C.byref(sh), C.byref(p), gp,
+                                       int(checkpoint_stride), _ptr(chi2), _ptr(grad_par), _ptr(grad_hits),
+                                       _ptr(grad_kfact), _ptr(grad_global), _stream(stream))
+        self._check(st, "p2r_forward_adjoint")
 
     def propagateToR(self, inErr, inPar, inChg, msP, outErr, outPar, bsize, nblk, trk_stride, hit_stride,
                      kfact=None, stream=None):

@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libp2r_amd.so")
 P2R_OK, P2R_ERR_ARG, P2R_ERR_SHAPE, P2R_ERR_HIP, P2R_ERR_UNSUPPORTED, P2R_ERR_NOMEM = range(6)
 P2R_REFERENCE, P2R_CHAINED = 0, 1
 P2R_EXACT, P2R_FAST = 0, 1
+P2R_GRAD_F32, P2R_GRAD_MIXED, P2R_GRAD_F64 = 0, 1, 2
 
 
 class p2r_shape(C.Structure):
@@ -47,6 +48,8 @@ def lib():
             "p2r_forward": ([vp, vp, vp, vp, C.POINTER(p2r_shape), C.c_int, C.POINTER(p2r_params), vp, vp], C.c_int),
             "p2r_forward_ragged": ([vp, vp, vp, vp, vp, i32, vp, i32, i32, C.c_int, C.POINTER(p2r_params), vp, i32,
                                     vp], C.c_int),
+            "p2r_forward_adjoint": ([vp, vp, vp, vp, C.POINTER(p2r_shape), C.POINTER(p2r_params), C.c_int, i32, vp,
+                                     vp, vp, vp, vp, vp], C.c_int),
             "p2r_propagateToR": ([vp, vp, vp, vp, vp, vp, vp, i32, i64, i64, i64, f32, vp], C.c_int),
             "p2r_KalmanUpdate": ([vp, vp, vp, vp, vp, i32, i64, i64, i64, vp], C.c_int),
             "p2r_gen_reference": ([vp, vp, C.POINTER(p2r_shape)], C.c_int),
