@@ -137,6 +137,8 @@ typedef enum p2r_grad_precision { P2R_GRAD_F32 = 0, P2R_GRAD_MIXED = 1, P2R_GRAD
  *               [4L+0..2] translation dx, dy, dz, [4L+3] rotation dphi about z
  *               (x' = x cos dphi - y sin dphi, ...) -- then [4*nlayer] the
  *               derivative with respect to b_scale (field = nominal * b_scale).
+ *               Tracks whose total chi2 is not finite (diverged fits) are left
+ *               out of these sums; their per-track outputs are still written.
  * The reference has no derivatives (this path is new). */
 int p2r_forward_adjoint(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* shape,
                         const p2r_params* params, int grad_precision, int32_t checkpoint_stride, float* chi2,

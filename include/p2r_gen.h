@@ -35,7 +35,8 @@ int p2r_reference_hit(int layer, float pos[3]);
  * regenerates identically and generation runs on all host threads.
  *
  * Generator modes.
- * P2R_GEN_CONSISTENT (default): each track's true state is base*(1+smear*N);
+ * P2R_GEN_CONSISTENT (default): each track's true state is base*(1+smear*N),
+ *   N truncated to |N| <= 3 (every track starts inside r = 19.5 cm);
  *   its hits lie on that track's own helix (charge q) at r_l, plus Gaussian
  *   measurement noise of variance hit_var per coordinate (the hit covariance
  *   given to the fit); the fit's input state is the true state plus noise
@@ -63,6 +64,7 @@ typedef struct p2r_gen_config {
 
 p2r_gen_config p2r_gen_default_config(void);
 
+/* nlayer 1..1024, ntrks % bsize == 0. */
 int p2r_gen_uniform(float* trk, float* hits, const p2r_shape* shape, const p2r_gen_config* cfg);
 
 /* Uneven batch ("ragged"): nevts events whose track counts are given in

@@ -316,6 +316,8 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_kernel(
     if (chi2) chi2[g] = (float)c2;
   }
   // ---- reverse sweep ----
+  // a track whose fit diverged (non-finite total chi2) adds nothing to the global sums
+  const bool in_global = active && isfinite((double)c2);
   const int nglob = 4 * nlayer + 1;
   const int wave = threadIdx.x >> 6, wl = threadIdx.x & 63;
   TrkT<RL> bar;
@@ -361,8 +363,8 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_kernel(
         }
         if (partials) {
           const double hx = h.pos[0], hy = h.pos[1];
-          const double a = active ? (double)hbar[0] : 0.0, b = active ? (double)hbar[1] : 0.0,
-                       cz = active ? (double)hbar[2] : 0.0;
+          const double a = in_global ? (double)hbar[0] : 0.0, b = in_global ? (double)hbar[1] : 0.0,
+                       cz = in_global ? (double)hbar[2] : 0.0;
           const double v0 = wave_sum(a), v1 = wave_sum(b), v2 = wave_sum(cz), v3 = wave_sum(-hy * a + hx * b);
           if (wl == 0) {
             lds_global[wave * nglob + 4 * l + 0] = v0;
@@ -385,7 +387,7 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_kernel(
   if (partials) {
     // b_scale: kfact_eff = kfact / b  =>  d/db = -kfact_eff * d/dkfact_eff at b = 1 scaled by 1/b; the
     // host passes kf = kfact/b, so d/db = -(kf / b) q kbar; written here for b folded into kf (b = 1 form)
-    const double vb = wave_sum(active ? -(double)kf * (double)q * (double)kbar : 0.0);
+    const double vb = wave_sum(in_global ? -(double)kf * (double)q * (double)kbar : 0.0);
     if (wl == 0) lds_global[wave * nglob + 4 * nlayer] = vb;
     __syncthreads();
     for (int j = threadIdx.x; j < nglob; j += kAdjThreads) {

@@ -59,6 +59,16 @@ inline double normal(uint64_t seed, uint64_t stream, uint64_t item, uint64_t fie
   const double u2 = uniform01(b);
   return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
 }
+// N(0,1) truncated to |N| <= 3 by redrawing (field + 64*k for the k-th redraw; 16 redraws never all miss
+// in practice, the last one is clamped).
+inline double normal_trunc3(uint64_t seed, uint64_t stream, uint64_t item, uint64_t field) {
+  double v = 0;
+  for (uint64_t k = 0; k < 16; ++k) {
+    v = normal(seed, stream, item, field + 64 * k);
+    if (std::fabs(v) <= 3.0) return v;
+  }
+  return std::fmax(-3.0, std::fmin(3.0, v));
+}
 inline uint64_t track_item(uint64_t ev, uint64_t tk) { return (ev << 32) | tk; }
 
 // Base state of the BASELINE configurations (propagate-tor-test_acc.v3.cpp:808-812).
@@ -203,13 +213,17 @@ bool helix_cylinder(const double p[6], double k, double r, double out[3]) {
 void fill_track(float* blk, int bs, int it, uint64_t item, const p2r_gen_config& c, double truth[6], int& q) {
   q = (counter(c.seed, kCharge, item, 0) >> 63) ? 1 : -1;
   std::memcpy(blk + 27 * bs + it, &q, sizeof(int));
-  for (int f = 0; f < 6; ++f) truth[f] = kBasePar[f] * (1.0 + c.smear * normal(c.seed, kPar, item, f));
   if (c.mode == P2R_GEN_LITERAL) {
+    for (int f = 0; f < 6; ++f) truth[f] = kBasePar[f] * (1.0 + c.smear * normal(c.seed, kPar, item, f));
     for (int f = 0; f < 6; ++f) blk[f * bs + it] = (float)truth[f];
     for (int f = 0; f < 21; ++f)
       blk[(6 + f) * bs + it] = (float)(kBaseCov[f] * (1.0 + c.smear * normal(c.seed, kCov, item, f)));
     return;
   }
+  // Truth smears truncated at 3 sigma: with the base position (r = 15 cm) every track then starts inside
+  // r = 19.5 cm, so the first cylinder (r0 = 20 cm by default) lies ahead of it.  Untruncated, about 8 in
+  // 1e6 tracks start beyond r0, their first propagation runs backwards and the fit diverges.
+  for (int f = 0; f < 6; ++f) truth[f] = kBasePar[f] * (1.0 + c.smear * normal_trunc3(c.seed, kPar, item, f));
   const Corr& corr = shrunk_correlation();
   double sig[6], z[6];
   for (int i = 0; i < 6; ++i) {
@@ -261,7 +275,7 @@ void fill_hit(float* hb, int bs, int it, uint64_t item, int layer, const LayerHi
 }
 
 bool shape_ok(const p2r_shape* s) {
-  return s && s->nevts >= 0 && s->ntrks >= 0 && s->nlayer >= 1 && s->nlayer <= 32 && s->bsize >= 1 &&
+  return s && s->nevts >= 0 && s->ntrks >= 0 && s->nlayer >= 1 && s->nlayer <= 1024 && s->bsize >= 1 &&
          s->bsize <= 1024 && s->ntrks % s->bsize == 0;
 }
 

@@ -178,3 +178,27 @@ def test_alignment_gradient_matches_finite_difference_of_total_chi2(orc, ctx):
              (total(bscale=1 + d) - total(bscale=1 - d)) / (2 * d)
         print(f"global[{idx}] ({kw}): adjoint {gg[idx]:.6e} FD {fd:.6e}", flush=True)
         assert abs(gg[idx] - fd) <= 1e-4 * abs(fd) + 1e-6, (kw, gg[idx], fd)
+
+
+def test_global_gradients_skip_diverged_tracks(ctx, batch):
+    """A track whose fit is not finite (here: NaN initial 1/pt) is left out of grad_global; the other
+    tracks' sums are unchanged (compared with the same batch where that track's hits are zeroed out of the
+    expected sum)."""
+    shape, trk, hits = batch
+    bad = 37
+    t2 = trk.copy()
+    T = t2.reshape(-1, 28, shape.bsize)
+    T[bad // shape.bsize, 3, bad % shape.bsize] = np.nan
+    _, c2, grads, gg = run_adjoint(ctx, t2, hits, shape, "f64", with_global=True)
+    assert not np.isfinite(c2[bad]) and np.isfinite(np.delete(c2, bad)).all()
+    assert np.isfinite(gg).all()
+    _, _, grads0, gg0 = run_adjoint(ctx, trk, hits, shape, "f64", with_global=True)
+    L, bs = shape.nlayer, shape.bsize
+    gh = grads0[bad, 6:6 + 3 * L].reshape(L, 3)
+    H = hits.reshape(-1, L, 9, bs)[bad // bs, :, :, bad % bs].astype(np.float64)
+    contrib = np.zeros(4 * L + 1)
+    contrib[0:4 * L:4], contrib[1:4 * L:4], contrib[2:4 * L:4] = gh[:, 0], gh[:, 1], gh[:, 2]
+    contrib[3:4 * L:4] = -H[:, 1] * gh[:, 0] + H[:, 0] * gh[:, 1]
+    import p2r_amd
+    contrib[4 * L] = -p2r_amd.default_params().kfact * grads0[bad, -1]
+    np.testing.assert_allclose(gg, gg0 - contrib, rtol=1e-6, atol=1e-9 * np.abs(gg0).max())
