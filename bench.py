@@ -309,54 +309,60 @@ def alignment_bench(ctx, args):
 def ragged_bench(ctx, args, uniform_chained, headline):
     """events with lognormal track counts (median 9600, sigma 0.5, clipped to
     [500, 40000]), hit count per track uniform in 5..20 (random subset of the 20 layers), chained
-    semantics, one persistent launch replayed from a captured HIP graph.  Rate = present-layer updates/s,
-    compared with the uniform batch of the headline shape in chained semantics."""
+    semantics, one launch replayed from a captured HIP graph.  Swept: Matriplex width 16/32/64 (the inputs
+    are generated in each width) x schedule (layer order; hit-count-sorted tiles of 1 or 2 rounds).  Rate =
+    present-layer updates/s, compared with the uniform batch of the headline shape in chained semantics."""
     import numpy as np
     import torch
 
     import p2r_amd
-    t0 = time.time()
     counts = p2r_amd.lognormal_counts(args.ragged_events, median=9600, sigma=0.5, lo=500, hi=40000)
-    trk, hits, mask, offs = p2r_amd.gen_ragged(counts, 20, args.bsize, kmin=5, kmax=20)
-    gen_s = time.time() - t0
-    updates = int(np.unpackbits(mask.view(np.uint8)).sum())
-    dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
-    dm = torch.from_numpy(mask.view(np.int32)).cuda()
-    do = torch.empty_like(dt)
-    del trk, hits
+    res = {"events": args.ragged_events, "tracks": int(counts.sum()), "sweep": {}}
+    best = {}
     stream = torch.cuda.Stream()
-    res = {"events": args.ragged_events, "tracks": int(counts.sum()), "slots": int(mask.size),
-           "present_updates": updates, "gen_s": gen_s, "tile_rounds": {}}
-    for prec in ("exact", "fast"):
-        best = None
-        for rounds in (1, 2, 4):
-            with torch.cuda.stream(stream):
-                for _ in range(2):
-                    ctx.forward_ragged(dt, dh, do, offs, dm, 20, args.bsize, "chained", precision=prec,
-                                       tile_rounds=rounds, stream=stream)
-                g = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(g, stream=stream):
-                    ctx.forward_ragged(dt, dh, do, offs, dm, 20, args.bsize, "chained", precision=prec,
-                                       tile_rounds=rounds, stream=stream)
-                g.replay()
-                stream.synchronize()
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record(stream)
-                for _ in range(5):
+    for bs in (16, 32, 64):
+        t0 = time.time()
+        trk, hits, mask, offs = p2r_amd.gen_ragged(counts, 20, bs, kmin=5, kmax=20)
+        gen_s = time.time() - t0
+        updates = int(np.unpackbits(mask.view(np.uint8)).sum())
+        res["present_updates"] = updates  # the same tracks and layer subsets in every width
+        res[f"slots_bs{bs}"] = int(mask.size)
+        dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
+        dm = torch.from_numpy(mask.view(np.int32)).cuda()
+        do = torch.empty_like(dt)
+        del trk, hits
+        for sched, rounds in (("layer", p2r_amd.P2R_RAGGED_LAYER_ORDER), ("sorted_r1", 1), ("sorted_r2", 2)):
+            for prec in ("exact", "fast"):
+                with torch.cuda.stream(stream):
+                    for _ in range(2):
+                        ctx.forward_ragged(dt, dh, do, offs, dm, 20, bs, "chained", precision=prec,
+                                           tile_rounds=rounds, stream=stream)
+                    g = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(g, stream=stream):
+                        ctx.forward_ragged(dt, dh, do, offs, dm, 20, bs, "chained", precision=prec,
+                                           tile_rounds=rounds, stream=stream)
                     g.replay()
-                e1.record(stream)
-            stream.synchronize()
-            ms = e0.elapsed_time(e1) / 5
-            rate = updates / (ms * 1e-3)
-            res["tile_rounds"][f"{prec}_r{rounds}"] = {"ms": ms, "updates_per_s": rate}
-            if best is None or rate > best[1]:
-                best = (rounds, rate, ms)
+                    stream.synchronize()
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    for _ in range(5):
+                        g.replay()
+                    e1.record(stream)
+                stream.synchronize()
+                ms = e0.elapsed_time(e1) / 5
+                rate = updates / (ms * 1e-3)
+                res["sweep"][f"{prec}_bs{bs}_{sched}"] = {"ms": ms, "updates_per_s": rate, "gen_s": gen_s}
+                if prec not in best or rate > best[prec][0]:
+                    best[prec] = (rate, ms, bs, sched)
+                del g
+        del dt, dh, dm, do
+        torch.cuda.empty_cache()
+    for prec, (rate, ms, bs, sched) in best.items():
         uni = uniform_chained.get(prec, {}).get("value")
         if uni is None and headline[0] == "chained" and headline[1] == prec:
             uni = headline[2]
-        res[prec] = {"best_tile_rounds": best[0], "ms": best[2], "updates_per_s": best[1],
-                     "uniform_chained_updates_per_s": uni,
-                     "fraction_of_uniform": (best[1] / uni) if uni else None}
+        res[prec] = {"best_bsize": bs, "best_schedule": sched, "ms": ms, "updates_per_s": rate,
+                     "uniform_chained_updates_per_s": uni, "fraction_of_uniform": (rate / uni) if uni else None}
     return res
 
 

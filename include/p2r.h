@@ -101,12 +101,19 @@ int p2r_forward(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_ou
  * layer_mask[slot]: bit L set when layer L has a hit; absent layers are skipped
  * (no propagation, no update); padding lanes carry mask 0 and are copied
  * through unchanged.  semantics / params / chi2 as in p2r_forward (chi2 indexed
- * by slot).  `tile_rounds` (1, 2 or 4; 0 = the tuned default) sets how many
- * 256-track rounds a workgroup's tile holds: tracks of a tile are ordered by hit
- * count before they are dealt to wavefronts, so larger tiles balance lanes
- * better and re-read more hit lines.  The reference has no uneven batches
+ * by slot).  `tile_rounds` picks the schedule:
+ *   P2R_RAGGED_LAYER_ORDER (-1): lanes in slot order; each wavefront walks the
+ *     union of its lanes' present layers and a lane updates at its own ones
+ *     (hit loads coalesced as in p2r_forward, lanes idle at layers they lack);
+ *   1, 2 or 4: persistent workgroups over tiles of 256*tile_rounds slots whose
+ *     tracks are ordered by hit count before they are dealt to wavefronts
+ *     (no idle lanes; each lane reads its own layers, so hit loads gather
+ *     from many cache lines);
+ *   P2R_RAGGED_AUTO (0): the library's choice (currently the layer order).
+ * The reference has no uneven batches
  * (every event has ntrks tracks and every track nlayer hits, tbb.cpp:15-34);
  * with all masks full and equal event sizes the result equals p2r_forward's. */
+enum { P2R_RAGGED_AUTO = 0, P2R_RAGGED_LAYER_ORDER = -1 };
 int p2r_forward_ragged(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out, const int64_t* blk_offsets,
                        int32_t nevts, const uint32_t* layer_mask, int32_t nlayer, int32_t bsize, int semantics,
                        const p2r_params* params, float* chi2, int32_t tile_rounds, void* stream);

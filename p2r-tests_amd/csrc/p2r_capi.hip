@@ -219,6 +219,71 @@ __global__ __launch_bounds__(kRagThreads) void fwd_ragged_kernel(const float* __
 }
 
 
+// Uneven batch, layer-ordered schedule: lanes keep slot order (consecutive
+// lanes = consecutive tracks of a Matriplex block), and each wavefront walks
+// the union of its lanes' present layers in increasing order; a lane updates
+// only at its own present layers.  Every hit load is then as coalesced as in
+// the uniform kernel; the price is idle lanes at layers other lanes of the
+// wavefront have and they lack.
+template <int SEM, bool CHI2, bool EXACT>
+__global__ __launch_bounds__(kThreads) void fwd_ragged_layer_kernel(const float* __restrict__ trk,
+                                                                    const float* __restrict__ hits,
+                                                                    float* __restrict__ out, float* __restrict__ chi2,
+                                                                    const uint32_t* __restrict__ mask, uint32_t nslot,
+                                                                    uint32_t bs, int nlayer, float kfact) {
+  const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
+  const bool active = g < nslot;
+  // bits at or above nlayer name no hit block: ignored
+  const uint32_t layer_bits = nlayer >= 32 ? ~0u : ((1u << nlayer) - 1u);
+  const uint32_t m = active ? (mask[g] & layer_bits) : 0u;
+  uint32_t wm = m;  // union over the wavefront (all 64 lanes take part in the shuffles)
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) wm |= (uint32_t)__shfl_xor((int)wm, o);
+  wm = __builtin_amdgcn_readfirstlane(wm);
+  if (!active) return;
+  const uint32_t blk = g / bs, lane = g - blk * bs;
+  const float* tb = trk + (size_t)blk * 28 * bs + lane;
+  Trk in;
+  load_trk(tb, bs, in);
+  const int q = reinterpret_cast<const int*>(tb)[(size_t)27 * bs];
+  const float k = (float)q * kfact;
+  const float* hb = hits + (size_t)blk * nlayer * 9 * bs + lane;
+  const size_t hstride = (size_t)9 * bs;
+  Trk cur = in;
+  float c2 = 0.f;
+  Hit hn;
+  uint32_t w = wm;
+  if (w && ((m >> __builtin_ctz(w)) & 1u)) load_hit(hb + (size_t)__builtin_ctz(w) * hstride, bs, hn);
+  while (w) {
+    const int l = __builtin_ctz(w);
+    w &= w - 1u;
+    const Hit h = hn;
+    if (w && ((m >> __builtin_ctz(w)) & 1u)) load_hit(hb + (size_t)__builtin_ctz(w) * hstride, bs, hn);
+    if ((m >> l) & 1u) {
+      Trk src;
+      if (SEM == P2R_REFERENCE) {
+        src = in;
+        opaque(src);
+      } else {
+        src = cur;
+      }
+      Trk prop;
+      Lane<EXACT>::propagate(src, k, h.pos[0], h.pos[1], prop);
+      float c = Lane<EXACT>::update(prop, h);
+      if (SEM == P2R_REFERENCE) {
+        opaque(prop);
+        opaque(c);
+      }
+      if (CHI2) c2 += c;
+      cur = prop;
+    }
+  }
+  float* ob = out + (size_t)blk * 28 * bs + lane;
+  store_trk(ob, bs, cur);
+  reinterpret_cast<int*>(ob)[(size_t)27 * bs] = q;
+  if (CHI2) chi2[g] = c2;
+}
+
 // ---------------------------------------------------------------------------
 // Forward + adjoint of the chained fit.  One lane per track.  Forward pass in
 // the state precision RS, checkpoints every CS layers (checkpoint 0 is the
@@ -500,6 +565,11 @@ int launch_ragged_t(p2r_ctx* c, hipStream_t st, const float* T, const float* H, 
 template <int SEM, bool CHI2, bool EXACT>
 int launch_ragged_r(int rounds, p2r_ctx* c, hipStream_t st, const float* T, const float* H, float* O, float* chi2,
                     const uint32_t* M, uint32_t nslot, uint32_t bs, int nlayer, float kf) {
+  if (rounds == P2R_RAGGED_LAYER_ORDER) {
+    const uint32_t grid = (nslot + kThreads - 1) / kThreads;
+    fwd_ragged_layer_kernel<SEM, CHI2, EXACT><<<grid, kThreads, 0, st>>>(T, H, O, chi2, M, nslot, bs, nlayer, kf);
+    return P2R_OK;
+  }
   switch (rounds) {
     case 1: return launch_ragged_t<SEM, CHI2, EXACT, 1>(c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
     case 2: return launch_ragged_t<SEM, CHI2, EXACT, 2>(c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
@@ -634,8 +704,9 @@ int p2r_forward_ragged(p2r_ctx* c, const void* trk_in, const void* hits, void* t
   if (nevts < 0 || nlayer < 1 || nlayer > 32 || bsize < 1 || bsize > 1024)
     return c->fail(P2R_ERR_SHAPE, "bad shape nevts=%d nlayer=%d bsize=%d (nlayer <= 32 for the 32-bit mask)", nevts,
                    nlayer, bsize);
-  if (tile_rounds != 0 && tile_rounds != 1 && tile_rounds != 2 && tile_rounds != 4)
-    return c->fail(P2R_ERR_ARG, "tile_rounds must be 0, 1, 2 or 4");
+  if (tile_rounds != P2R_RAGGED_AUTO && tile_rounds != P2R_RAGGED_LAYER_ORDER && tile_rounds != 1 &&
+      tile_rounds != 2 && tile_rounds != 4)
+    return c->fail(P2R_ERR_ARG, "tile_rounds must be P2R_RAGGED_AUTO, P2R_RAGGED_LAYER_ORDER, 1, 2 or 4");
   if (blk_offsets[0] != 0) return c->fail(P2R_ERR_SHAPE, "blk_offsets[0] must be 0");
   for (int32_t e = 0; e < nevts; ++e)
     if (blk_offsets[e + 1] < blk_offsets[e]) return c->fail(P2R_ERR_SHAPE, "blk_offsets decrease at event %d", e);
@@ -645,7 +716,7 @@ int p2r_forward_ragged(p2r_ctx* c, const void* trk_in, const void* hits, void* t
   if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
   if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
   if (nslot == 0) return P2R_OK;
-  const int rounds = tile_rounds ? tile_rounds : 2;
+  const int rounds = tile_rounds != P2R_RAGGED_AUTO ? tile_rounds : P2R_RAGGED_LAYER_ORDER;
   DeviceGuard g(c->device);
   hipStream_t st = static_cast<hipStream_t>(stream);
   auto T = static_cast<const float*>(trk_in);

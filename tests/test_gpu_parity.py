@@ -251,7 +251,7 @@ def run_ragged(ctx, trk, hits, mask, offs, nlayer, bsize, semantics, precision="
     return do.cpu().numpy(), dc.cpu().numpy()
 
 
-@pytest.mark.parametrize("rounds", [1, 2, 4])
+@pytest.mark.parametrize("rounds", [-1, 1, 2, 4])
 @pytest.mark.parametrize("semantics", ["chained", "reference"])
 def test_exact_ragged_matches_oracle(orc, ctx, semantics, rounds):
     """Uneven events + masked layers, one persistent launch: bit-identical to the oracle's masked run."""
@@ -268,6 +268,18 @@ def test_exact_ragged_matches_oracle(orc, ctx, semantics, rounds):
     assert n == 0 and nq == 0 and nc == 0
 
 
+@pytest.mark.parametrize("rounds", [-1, 1])
+@pytest.mark.parametrize("bsize", [1, 8, 64])
+def test_exact_ragged_other_widths(orc, ctx, bsize, rounds):
+    """Matriplex widths other than 32 through both schedules (chained, chi2 on)."""
+    counts, trk, hits, mask, offs = _ragged_inputs(nevts=12, bsize=bsize)
+    nblk = int(offs[-1])
+    gpu, c2g = run_ragged(ctx, trk, hits, mask, offs, 20, bsize, "chained", rounds=rounds)
+    cpu, c2c = orc.run_f32(trk, hits, nblk, 20, bsize, semantics=1, chi2=True, mask=mask, cr_trig=True)
+    n = bit_mismatches(gpu.reshape(-1, 28, bsize)[:, :27], cpu.reshape(-1, 28, bsize)[:, :27])
+    assert n == 0 and bit_mismatches(c2g, c2c) == 0
+
+
 def test_ragged_full_masks_equal_uniform(ctx):
     """All layers present and equal event sizes: the ragged launch reproduces p2r_forward bit for bit."""
     import p2r_amd
@@ -281,10 +293,11 @@ def test_ragged_full_masks_equal_uniform(ctx):
         assert bit_mismatches(uni, rag) == 0
 
 
-def test_ragged_ignores_mask_bits_beyond_nlayer(orc, ctx):
+@pytest.mark.parametrize("rounds", [-1, 1])
+def test_ragged_ignores_mask_bits_beyond_nlayer(orc, ctx, rounds):
     counts, trk, hits, mask, offs = _ragged_inputs(nevts=8, nlayer=12)
     junk = mask | np.uint32(0xFFFFF000)  # bits 12..31 name no hit block
-    gpu, _ = run_ragged(ctx, trk, hits, junk, offs, 12, 32, "chained")
+    gpu, _ = run_ragged(ctx, trk, hits, junk, offs, 12, 32, "chained", rounds=rounds)
     cpu = orc.run_f32(trk, hits, int(offs[-1]), 12, 32, semantics=1, mask=mask, cr_trig=True)
     assert bit_mismatches(gpu.reshape(-1, 28, 32)[:, :27], cpu.reshape(-1, 28, 32)[:, :27]) == 0
 
