@@ -143,3 +143,15 @@ def test_create_without_device_returns_error():
     lib = p2r_amd.lib()
     assert lib.p2r_forward(None, None, None, None, None, 0, None, None, None) == p2r_amd.P2R_ERR_ARG
     assert lib.p2r_error_string(2) == b"unsupported shape"
+
+
+def test_torch_extension_loads_and_exports():
+    """The thin torch extension is built in-tree and exposes its three functions (no GPU call)."""
+    import os
+
+    import p2r_amd.autograd as ag
+    assert os.path.dirname(ag._ext.__file__).endswith(os.path.join("p2r-tests_amd", "p2r_amd"))
+    for name in ("forward", "forward_adjoint", "default_kfact"):
+        assert hasattr(ag._ext, name)
+    import p2r_amd
+    assert ag._ext.default_kfact() == p2r_amd.default_params().kfact
