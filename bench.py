@@ -248,16 +248,22 @@ def adjoint_bench(ctx, args, variant_ms, headline):
     gpar = torch.empty(shape.nblk * 6 * shape.bsize, device="cuda")
     ghit = torch.empty(shape.nblk * shape.nlayer * 3 * shape.bsize, device="cuda")
     gk = torch.empty(n, device="cuda")
-    res = {"shape": list(shape.tuple()), "checkpoint_stride": 4}
+    res = {"shape": list(shape.tuple()), "checkpoint_stride": 4, "stride_sweep": {}}
     for prec in ("exact", "fast"):
         fwd_ms = variant_ms.get(("chained", prec))
         if fwd_ms is None and headline[0] == "chained" and headline[1] == prec:
             fwd_ms = headline[2]
         for gp in ("f32", "mixed", "f64"):
-            ms = _timed(lambda: ctx.forward_adjoint(d_trk, d_hit, d_out, shape, grad_precision=gp, precision=prec,
-                                                    chi2=c2, grad_par=gpar, grad_hits=ghit, grad_kfact=gk))
-            res[f"{prec}_{gp}"] = {"ms": ms, "updates_per_s": shape.ntrk_total * shape.nlayer / (ms * 1e-3),
-                                   "ratio_to_chained_forward": (ms / fwd_ms) if fwd_ms else None}
+            for cs in ((4, 1, 2, 8) if gp != "f64" else (4, 1)):
+                ms = _timed(lambda: ctx.forward_adjoint(d_trk, d_hit, d_out, shape, grad_precision=gp, precision=prec,
+                                                        checkpoint_stride=cs, chi2=c2, grad_par=gpar, grad_hits=ghit,
+                                                        grad_kfact=gk))
+                r = {"ms": ms, "updates_per_s": shape.ntrk_total * shape.nlayer / (ms * 1e-3),
+                     "ratio_to_chained_forward": (ms / fwd_ms) if fwd_ms else None}
+                if cs == 4:
+                    res[f"{prec}_{gp}"] = r
+                else:
+                    res["stride_sweep"][f"{prec}_{gp}_cs{cs}"] = r
     return res
 
 
@@ -299,9 +305,10 @@ def alignment_bench(ctx, args):
         res[gp]["rel_err_vs_f64_norm"] = float(np.linalg.norm(grads[gp] - ref) / np.linalg.norm(ref))
         res[gp]["rel_err_vs_f64_max_component"] = float(np.max(np.abs(grads[gp] - ref) / (np.abs(ref) + 1e-30)))
     g = ref[:4 * nl].reshape(nl, 4)
-    # the gradient at nominal points along the hidden misalignment: correlation of -grad with the truth
-    res["corr_minus_grad_vs_truth"] = {k: float(np.corrcoef(-g[:, i], offsets[:, i])[0, 1])
-                                       for i, k in enumerate(("dx", "dy", "dz", "dphi"))}
+    # the hits carry the hidden motion, so the correction that undoes it is -offsets; a descent step -grad
+    # points along it when grad correlates positively with the offsets
+    res["corr_grad_vs_hidden_offsets"] = {k: float(np.corrcoef(g[:, i], offsets[:, i])[0, 1])
+                                          for i, k in enumerate(("dx", "dy", "dz", "dphi"))}
     res["d_chi2_d_bscale"] = float(ref[-1])
     return res
 

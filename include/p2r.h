@@ -129,8 +129,11 @@ typedef enum p2r_grad_precision { P2R_GRAD_F32 = 0, P2R_GRAD_MIXED = 1, P2R_GRAD
  * (P2R_CHAINED semantics, uniform batch as in p2r_forward).  Objective per
  * track: the sum over layers of the local chi2 r^T S^-1 r of each update.
  * Memory: the track state is checkpointed every `checkpoint_stride` layers
- * (0 = 4) in context workspace; the reverse sweep recomputes each segment
- * from its checkpoint instead of storing every intermediate.
+ * (1, 2, 4 or 8; 0 = 4) in context workspace; the reverse sweep recomputes
+ * each segment from its checkpoint instead of storing every intermediate
+ * (stride 1 stores every layer's input state and recomputes nothing but the
+ * layer's own intermediates).  Workspace: (ceil(nlayer/stride) - 1) * 27 *
+ * ntrk states of 4 bytes (P2R_GRAD_F32) or 8 bytes.
  * Outputs (device memory; any may be NULL except trk_out):
  *   trk_out     final fitted tracks (same as p2r_forward, P2R_CHAINED)
  *   chi2        float[ntrk]: total chi2 per track (flat track index)

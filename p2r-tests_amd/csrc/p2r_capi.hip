@@ -746,7 +746,8 @@ int p2r_forward_adjoint(p2r_ctx* c, const void* trk_in, const void* hits, void* 
   if (grad_precision < P2R_GRAD_F32 || grad_precision > P2R_GRAD_F64)
     return c->fail(P2R_ERR_ARG, "bad grad_precision %d", grad_precision);
   const int cs = checkpoint_stride ? checkpoint_stride : 4;
-  if (cs != 2 && cs != 4 && cs != 8) return c->fail(P2R_ERR_ARG, "checkpoint_stride must be 0, 2, 4 or 8");
+  if (cs != 1 && cs != 2 && cs != 4 && cs != 8)
+    return c->fail(P2R_ERR_ARG, "checkpoint_stride must be 0, 1, 2, 4 or 8");
   const uint64_t ntrk = (uint64_t)s->nevts * (uint64_t)s->ntrks;
   if (ntrk >= (1ull << 31)) return c->fail(P2R_ERR_SHAPE, "too many tracks");
   const p2r_params p = params ? *params : p2r_default_params();
@@ -775,9 +776,11 @@ int p2r_forward_adjoint(p2r_ctx* c, const void* trk_in, const void* hits, void* 
   launch_adjoint_p<E, CS>(grad_precision, grid, lds, st, T, H, O, chi2, grad_par, grad_hits, grad_kfact, partials, ws, \
                           n, bs, nlayer, kf)
   if (exact) {
-    if (cs == 2) P2R_ADJ(true, 2); else if (cs == 4) P2R_ADJ(true, 4); else P2R_ADJ(true, 8);
+    if (cs == 1) P2R_ADJ(true, 1); else if (cs == 2) P2R_ADJ(true, 2); else if (cs == 4) P2R_ADJ(true, 4);
+    else P2R_ADJ(true, 8);
   } else {
-    if (cs == 2) P2R_ADJ(false, 2); else if (cs == 4) P2R_ADJ(false, 4); else P2R_ADJ(false, 8);
+    if (cs == 1) P2R_ADJ(false, 1); else if (cs == 2) P2R_ADJ(false, 2); else if (cs == 4) P2R_ADJ(false, 4);
+    else P2R_ADJ(false, 8);
   }
 #undef P2R_ADJ
   int rc = c->check_launch("p2r_forward_adjoint");

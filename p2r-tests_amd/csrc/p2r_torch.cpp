@@ -6,8 +6,10 @@
 // torch's caching allocator and forwards device pointers.  Used by
 // p2r_amd.autograd (the torch.autograd.Function of the chained fit).
 #include <torch/extension.h>
-#include <c10/hip/HIPGuard.h>
-#include <c10/hip/HIPStream.h>
+// ROCm builds of torch give device tensors the device type "cuda": the guard and
+// stream accessors are the HIP implementations registered under that type.
+#include <ATen/hip/impl/HIPGuardImplMasqueradingAsCUDA.h>
+#include <ATen/hip/impl/HIPStreamMasqueradingAsCUDA.h>
 
 #include <mutex>
 #include <vector>
@@ -68,12 +70,12 @@ std::vector<at::Tensor> forward(const at::Tensor& trk, const at::Tensor& hits, i
                                 bool with_chi2) {
   const p2r_shape s = make_shape(nevts, ntrks, nlayer, bsize);
   check_buffers(trk, hits, s);
-  const c10::hip::HIPGuard guard(trk.device());
+  const c10::hip::HIPGuardMasqueradingAsCUDA guard(trk.device());
   p2r_ctx* c = ctx_for(trk.device().index());
   at::Tensor out = at::empty_like(trk);
   at::Tensor chi2 = with_chi2 ? at::empty({(int64_t)s.nevts * s.ntrks}, trk.options()) : at::Tensor();
   const p2r_params p = make_params(kfact, precision);
-  hipStream_t st = c10::hip::getCurrentHIPStream(trk.device().index()).stream();
+  hipStream_t st = c10::hip::getCurrentHIPStreamMasqueradingAsCUDA(trk.device().index()).stream();
   check(p2r_forward(c, trk.data_ptr(), hits.data_ptr(), out.data_ptr(), &s, (int)semantics, &p,
                     with_chi2 ? chi2.data_ptr<float>() : nullptr, st),
         c, "p2r_forward");
@@ -88,7 +90,7 @@ std::vector<at::Tensor> forward_adjoint(const at::Tensor& trk, const at::Tensor&
                                         int64_t grad_precision, int64_t checkpoint_stride, bool with_global) {
   const p2r_shape s = make_shape(nevts, ntrks, nlayer, bsize);
   check_buffers(trk, hits, s);
-  const c10::hip::HIPGuard guard(trk.device());
+  const c10::hip::HIPGuardMasqueradingAsCUDA guard(trk.device());
   p2r_ctx* c = ctx_for(trk.device().index());
   const int64_t nblk = (int64_t)s.nevts * (s.ntrks / s.bsize);
   at::Tensor out = at::empty_like(trk);
@@ -99,7 +101,7 @@ std::vector<at::Tensor> forward_adjoint(const at::Tensor& trk, const at::Tensor&
   at::Tensor gg = with_global ? at::empty({4 * (int64_t)s.nlayer + 1}, trk.options().dtype(at::kDouble))
                               : at::Tensor();
   const p2r_params p = make_params(kfact, precision);
-  hipStream_t st = c10::hip::getCurrentHIPStream(trk.device().index()).stream();
+  hipStream_t st = c10::hip::getCurrentHIPStreamMasqueradingAsCUDA(trk.device().index()).stream();
   check(p2r_forward_adjoint(c, trk.data_ptr(), hits.data_ptr(), out.data_ptr(), &s, &p, (int)grad_precision,
                             (int32_t)checkpoint_stride, chi2.data_ptr<float>(), gpar.data_ptr<float>(),
                             ghit.data_ptr<float>(), gk.data_ptr<float>(),

@@ -22,8 +22,8 @@ setup(
             define_macros=[("__HIP_PLATFORM_AMD__", "1"), ("USE_ROCM", "1")],
             library_dirs=[os.path.join(HERE, "lib")],
             libraries=["p2r_amd", "c10_hip"],
-            extra_compile_args=["-O2", "-std=c++17"],
-            extra_link_args=["-Wl,-rpath,$ORIGIN/../lib"],
+            extra_compile_args=["-O2", "-std=c++17", "-g0"],
+            extra_link_args=["-Wl,-rpath,$ORIGIN/../lib", "-s"],
         )
     ],
     cmdclass={"build_ext": BuildExtension},

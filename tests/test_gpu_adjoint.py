@@ -128,7 +128,7 @@ def test_adjoint_fp32_gradients_vs_fp64(orc, ctx, batch, gp, median_bound, max_b
 def test_checkpoint_stride_does_not_change_gradients(ctx, batch):
     shape, trk, hits = batch
     ref = run_adjoint(ctx, trk, hits, shape, "f32", cs=4)[2]
-    for cs in (2, 8):
+    for cs in (1, 2, 8):
         g = run_adjoint(ctx, trk, hits, shape, "f32", cs=cs)[2]
         assert np.array_equal(g.view(np.uint64), ref.view(np.uint64)), cs
 
