@@ -57,6 +57,7 @@ def main():
     ap.add_argument("--no-alignment", action="store_true", help="skip the alignment (config 5) measurement")
     ap.add_argument("--alignment-tracks", type=int, default=1000000)
     ap.add_argument("--no-parity", action="store_true", help="skip the parity spot check")
+    ap.add_argument("--no-autotune", action="store_true", help="time the library's default launch choices")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (steps times) and exit: the "
                          "reference kernel for scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -98,6 +99,13 @@ def main():
 
     def step(sem=args.semantics, prec=args.precision):
         ctx.forward(d_trk, d_hit, d_out, shape, sem, stream=stream, precision=prec)
+
+    # launch choices for this shape, measured on these buffers (workgroup size x layers per launch)
+    tuning = None
+    if not args.no_autotune:
+        rep = ctx.autotune_forward(d_trk, d_hit, d_out, shape, args.semantics, precision=args.precision, reps=5,
+                                   stream=stream)
+        tuning = {"chosen": ctx.get_tuning().as_dict(), "candidates": rep.as_list()}
 
     for _ in range(args.warmup):
         step()
@@ -211,6 +219,7 @@ def main():
                      "hbm_frac": bytes_per_launch / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
                      "kernel": "fwd_uniform_kernel"},
         "cpu_baseline": cpu,
+        "autotune": tuning,
         "variants": variants,
         "adjoint": adjoint,
         "ragged": ragged,
@@ -316,59 +325,63 @@ def alignment_bench(ctx, args):
 def ragged_bench(ctx, args, uniform_chained, headline):
     """events with lognormal track counts (median 9600, sigma 0.5, clipped to
     [500, 40000]), hit count per track uniform in 5..20 (random subset of the 20 layers), chained
-    semantics, one launch replayed from a captured HIP graph.  Swept: Matriplex width 16/32/64 (the inputs
-    are generated in each width) x schedule (layer order; hit-count-sorted tiles of 1 or 2 rounds).  Rate =
-    present-layer updates/s, compared with the uniform batch of the headline shape in chained semantics."""
+    semantics, one launch replayed from a captured HIP graph.  For each Matriplex width 16/32/64 (inputs
+    generated in that width) the autotuner times the schedules (layer order at 64/128/256 lanes per
+    workgroup, hit-count-sorted tiles of 1/2/4 rounds); the chosen one is then captured and replayed.
+    Rate = present-layer updates/s, compared with the uniform batch of the headline shape, chained."""
     import numpy as np
     import torch
 
     import p2r_amd
     counts = p2r_amd.lognormal_counts(args.ragged_events, median=9600, sigma=0.5, lo=500, hi=40000)
-    res = {"events": args.ragged_events, "tracks": int(counts.sum()), "sweep": {}}
+    res = {"events": args.ragged_events, "tracks": int(counts.sum()), "widths": {}}
     best = {}
     stream = torch.cuda.Stream()
+    keep = ctx.get_tuning()
     for bs in (16, 32, 64):
         t0 = time.time()
         trk, hits, mask, offs = p2r_amd.gen_ragged(counts, 20, bs, kmin=5, kmax=20)
         gen_s = time.time() - t0
         updates = int(np.unpackbits(mask.view(np.uint8)).sum())
         res["present_updates"] = updates  # the same tracks and layer subsets in every width
-        res[f"slots_bs{bs}"] = int(mask.size)
         dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
         dm = torch.from_numpy(mask.view(np.int32)).cuda()
         do = torch.empty_like(dt)
         del trk, hits
-        for sched, rounds in (("layer", p2r_amd.P2R_RAGGED_LAYER_ORDER), ("sorted_r1", 1), ("sorted_r2", 2)):
-            for prec in ("exact", "fast"):
-                with torch.cuda.stream(stream):
-                    for _ in range(2):
-                        ctx.forward_ragged(dt, dh, do, offs, dm, 20, bs, "chained", precision=prec,
-                                           tile_rounds=rounds, stream=stream)
-                    g = torch.cuda.CUDAGraph()
-                    with torch.cuda.graph(g, stream=stream):
-                        ctx.forward_ragged(dt, dh, do, offs, dm, 20, bs, "chained", precision=prec,
-                                           tile_rounds=rounds, stream=stream)
-                    g.replay()
-                    stream.synchronize()
-                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                    e0.record(stream)
-                    for _ in range(5):
-                        g.replay()
-                    e1.record(stream)
+        for prec in ("exact", "fast"):
+            with torch.cuda.stream(stream):
+                rep = ctx.autotune_ragged(dt, dh, do, offs, dm, 20, bs, "chained", precision=prec, reps=3,
+                                          stream=stream)
+                chosen = ctx.get_tuning()
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=stream):
+                    ctx.forward_ragged(dt, dh, do, offs, dm, 20, bs, "chained", precision=prec, stream=stream)
+                g.replay()
                 stream.synchronize()
-                ms = e0.elapsed_time(e1) / 5
-                rate = updates / (ms * 1e-3)
-                res["sweep"][f"{prec}_bs{bs}_{sched}"] = {"ms": ms, "updates_per_s": rate, "gen_s": gen_s}
-                if prec not in best or rate > best[prec][0]:
-                    best[prec] = (rate, ms, bs, sched)
-                del g
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(5):
+                    g.replay()
+                e1.record(stream)
+            stream.synchronize()
+            ms = e0.elapsed_time(e1) / 5
+            rate = updates / (ms * 1e-3)
+            res["widths"][f"{prec}_bs{bs}"] = {
+                "graph_ms": ms, "updates_per_s": rate, "slots": int(mask.size), "gen_s": gen_s,
+                "schedule": chosen.ragged_schedule, "threads": chosen.ragged_threads,
+                "candidates": [{"schedule": e["schedule"], "threads": e["threads"], "ms": e["ms"]}
+                               for e in rep.as_list()]}
+            if prec not in best or rate > best[prec][0]:
+                best[prec] = (rate, ms, bs, chosen.ragged_schedule, chosen.ragged_threads)
+            del g
+            ctx.set_tuning(keep)
         del dt, dh, dm, do
         torch.cuda.empty_cache()
-    for prec, (rate, ms, bs, sched) in best.items():
+    for prec, (rate, ms, bs, sched, thr) in best.items():
         uni = uniform_chained.get(prec, {}).get("value")
         if uni is None and headline[0] == "chained" and headline[1] == prec:
             uni = headline[2]
-        res[prec] = {"best_bsize": bs, "best_schedule": sched, "ms": ms, "updates_per_s": rate,
+        res[prec] = {"best_bsize": bs, "best_schedule": sched, "best_threads": thr, "ms": ms, "updates_per_s": rate,
                      "uniform_chained_updates_per_s": uni, "fraction_of_uniform": (rate / uni) if uni else None}
     return res
 

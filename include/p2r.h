@@ -171,6 +171,71 @@ int p2r_propagateToR(p2r_ctx* ctx, const float* inErr, const float* inPar, const
 int p2r_KalmanUpdate(p2r_ctx* ctx, float* trkErr, float* inPar, const float* hitErr, const float* msP,
                      int32_t bsize, int64_t nblk, int64_t trk_stride, int64_t hit_stride, void* stream);
 
+/* ---- Launch choices and the autotuner ---------------------------------------
+ * The reference fixes Matriplex width, threads per block and layers per kernel
+ * at compile time (build.py:68-80 sweeps them by rebuilding; hip_local.cu:911
+ * templates the kernel on <bsize, layers>).  Here they are runtime choices held
+ * by the context, set by hand or measured by the autotuners below. */
+typedef struct p2r_tuning {
+  int32_t fwd_threads;           /* workgroup size of p2r_forward: 64, 128 or 256 (default 256) */
+  int32_t fwd_layers_per_launch; /* 0 = all layers in one launch (default); n = launches of n layers,
+                                    the state passing through trk_out in between */
+  int32_t ragged_schedule;       /* what P2R_RAGGED_AUTO runs: P2R_RAGGED_LAYER_ORDER (default), 1, 2, 4 */
+  int32_t ragged_threads;        /* workgroup size of the layer-ordered schedule: 64, 128 or 256 (256) */
+  int32_t checkpoint_stride;     /* what checkpoint_stride = 0 means in p2r_forward_adjoint: 1, 2, 4, 8 (4) */
+} p2r_tuning;
+int p2r_get_tuning(const p2r_ctx* ctx, p2r_tuning* out);
+int p2r_set_tuning(p2r_ctx* ctx, const p2r_tuning* in); /* P2R_ERR_ARG on a value outside the lists above */
+
+/* One timed candidate: the launch choices and the mean time of one call (ms). */
+typedef struct p2r_tune_entry {
+  int32_t bsize, threads, layers_per_launch, schedule, checkpoint_stride;
+  float ms;
+} p2r_tune_entry;
+typedef struct p2r_tune_report {
+  int32_t count; /* entries used */
+  int32_t best;  /* index of the fastest */
+  p2r_tune_entry entry[32];
+} p2r_tune_report;
+
+/* Autotuners.  Each times its candidates on the caller's buffers (2 untimed
+ * calls, then the mean of `reps` calls, HIP events on `stream`), stores the
+ * fastest in the context's tuning and fills `report` (nullable).  They
+ * synchronize the stream and create events: not for use under graph capture.
+ * Outputs are overwritten with valid results of the last candidate run.
+ *   forward: fwd_threads {64, 128, 256} x fwd_layers_per_launch {all,
+ *            ceil(nlayer/2), ceil(nlayer/4)};
+ *   ragged:  ragged_schedule {layer order x ragged_threads {64, 128, 256},
+ *            sorted tiles of 1, 2, 4 rounds};
+ *   adjoint: checkpoint_stride {1, 2, 4, 8} (grad_precision as given). */
+int p2r_autotune_forward(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* shape,
+                         int semantics, const p2r_params* params, int32_t reps, p2r_tune_report* report,
+                         void* stream);
+int p2r_autotune_ragged(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out, const int64_t* blk_offsets,
+                        int32_t nevts, const uint32_t* layer_mask, int32_t nlayer, int32_t bsize, int semantics,
+                        const p2r_params* params, int32_t reps, p2r_tune_report* report, void* stream);
+int p2r_autotune_adjoint(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* shape,
+                         const p2r_params* params, int grad_precision, float* chi2, float* grad_par,
+                         float* grad_hits, float* grad_kfact, double* grad_global, int32_t reps,
+                         p2r_tune_report* report, void* stream);
+
+/* Matriplex width.  The width is the caller's data layout, so it is not a
+ * context setting: p2r_repack converts a batch between widths on the device,
+ * and p2r_autotune_width times p2r_forward (current tuning) on copies of the
+ * caller's batch repacked to each width in `widths` (each must divide ntrks;
+ * at most 32), reporting the fastest width.  It allocates its copies with
+ * hipMalloc and frees them before returning.
+ *   p2r_repack: `nsub` sub-blocks of `words` words per lane (MPTRK: nsub 1,
+ *   words 28; MPHIT: nsub nlayer, words 9) for nevts events of ntrks tracks;
+ *   track t of event e moves from (block e*ntrks/src_bsize + t/src_bsize,
+ *   lane t%src_bsize) to the same place in the dst_bsize layout.  src and dst
+ *   must not overlap. */
+int p2r_repack(p2r_ctx* ctx, const void* src, int32_t src_bsize, void* dst, int32_t dst_bsize, int32_t nevts,
+               int32_t ntrks, int32_t nsub, int32_t words, void* stream);
+int p2r_autotune_width(p2r_ctx* ctx, const void* trk_in, const void* hits, const p2r_shape* shape, int semantics,
+                       const p2r_params* params, const int32_t* widths, int32_t nwidths, int32_t reps,
+                       p2r_tune_report* report, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,12 +44,15 @@ __device__ __forceinline__ void load_hit(const float* hb, uint32_t bs, Hit& h) {
 
 // Whole batch, all layers per lane.  SEM: P2R_REFERENCE / P2R_CHAINED;
 // EXACT selects the per-lane math instance (p2r_lane.hpp).
+// Layers [l0, l1) of nlayer (all of them for one launch; a layer range when the
+// tuning splits the loop over several launches: then chi2 accumulates if acc).
+// Workgroups of 64..256 lanes (blockDim.x).
 template <int SEM, bool CHI2, bool EXACT>
-__global__ __launch_bounds__(kThreads) void fwd_uniform_kernel(const float* __restrict__ trk,
-                                                               const float* __restrict__ hits,
-                                                               float* __restrict__ out, float* __restrict__ chi2,
-                                                               uint32_t ntrk, uint32_t bs, int nlayer, float kfact) {
-  const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
+__global__ __launch_bounds__(kThreads) void fwd_uniform_kernel(const float* trk, const float* __restrict__ hits,
+                                                               float* out, float* __restrict__ chi2, uint32_t ntrk,
+                                                               uint32_t bs, int nlayer, float kfact, int l0, int l1,
+                                                               bool acc) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ntrk) return;
   const uint32_t blk = g / bs, lane = g - blk * bs;
   const float* tb = trk + (size_t)blk * 28 * bs + lane;
@@ -63,10 +66,10 @@ __global__ __launch_bounds__(kThreads) void fwd_uniform_kernel(const float* __re
   Trk cur = in;
   float c2 = 0.f;
   Hit hn;
-  load_hit(hb, bs, hn);
-  for (int l = 0; l < nlayer; ++l) {
+  load_hit(hb + (size_t)l0 * hstride, bs, hn);
+  for (int l = l0; l < l1; ++l) {
     const Hit h = hn;
-    if (l + 1 < nlayer) load_hit(hb + (l + 1) * hstride, bs, hn);
+    if (l + 1 < l1) load_hit(hb + (l + 1) * hstride, bs, hn);
     Trk src;
     if (SEM == P2R_REFERENCE) {
       src = in;      // tbb.cpp:894: every layer starts from the input track
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(kThreads) void fwd_uniform_kernel(const float* __re
   float* ob = out + (size_t)blk * 28 * bs + lane;
   store_trk(ob, bs, cur);
   reinterpret_cast<int*>(ob)[(size_t)27 * bs] = q;
-  if (CHI2) chi2[g] = c2;
+  if (CHI2) chi2[g] = acc ? chi2[g] + c2 : c2;
 }
 
 
@@ -231,7 +234,7 @@ __global__ __launch_bounds__(kThreads) void fwd_ragged_layer_kernel(const float*
                                                                     float* __restrict__ out, float* __restrict__ chi2,
                                                                     const uint32_t* __restrict__ mask, uint32_t nslot,
                                                                     uint32_t bs, int nlayer, float kfact) {
-  const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;  // workgroups of 64..256 lanes
   const bool active = g < nslot;
   // bits at or above nlayer name no hit block: ignored
   const uint32_t layer_bits = nlayer >= 32 ? ~0u : ((1u << nlayer) - 1u);
@@ -530,20 +533,42 @@ __global__ __launch_bounds__(kThreads) void kalman_blocks_kernel(float* __restri
 inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 inline unsigned grid_for_n(uint64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
+struct UniformLaunch {
+  const float* T;
+  const float* H;
+  float* O;
+  float* chi2;
+  uint32_t n, bs;
+  int nlayer;
+  float kf;
+};
 template <int SEM, bool CHI2>
-void launch_uniform_p(bool exact, dim3 grid, dim3 block, hipStream_t st, const float* T, const float* H, float* O,
-                      float* chi2, uint32_t n, uint32_t bs, int nlayer, float kf) {
-  if (exact) fwd_uniform_kernel<SEM, CHI2, true><<<grid, block, 0, st>>>(T, H, O, chi2, n, bs, nlayer, kf);
-  else fwd_uniform_kernel<SEM, CHI2, false><<<grid, block, 0, st>>>(T, H, O, chi2, n, bs, nlayer, kf);
+void launch_uniform_p(bool exact, dim3 grid, dim3 block, hipStream_t st, const UniformLaunch& a, const float* in,
+                      int l0, int l1, bool acc) {
+  if (exact)
+    fwd_uniform_kernel<SEM, CHI2, true><<<grid, block, 0, st>>>(in, a.H, a.O, a.chi2, a.n, a.bs, a.nlayer, a.kf, l0,
+                                                                l1, acc);
+  else
+    fwd_uniform_kernel<SEM, CHI2, false><<<grid, block, 0, st>>>(in, a.H, a.O, a.chi2, a.n, a.bs, a.nlayer, a.kf, l0,
+                                                                 l1, acc);
 }
-void launch_uniform(int sem, bool chi2_on, bool exact, dim3 grid, dim3 block, hipStream_t st, const float* T,
-                    const float* H, float* O, float* chi2, uint32_t n, uint32_t bs, int nlayer, float kf) {
-  if (sem == P2R_REFERENCE) {
-    if (chi2_on) launch_uniform_p<P2R_REFERENCE, true>(exact, grid, block, st, T, H, O, chi2, n, bs, nlayer, kf);
-    else launch_uniform_p<P2R_REFERENCE, false>(exact, grid, block, st, T, H, O, chi2, n, bs, nlayer, kf);
-  } else {
-    if (chi2_on) launch_uniform_p<P2R_CHAINED, true>(exact, grid, block, st, T, H, O, chi2, n, bs, nlayer, kf);
-    else launch_uniform_p<P2R_CHAINED, false>(exact, grid, block, st, T, H, O, chi2, n, bs, nlayer, kf);
+// The whole forward: one launch over all layers, or launches of `per_launch`
+// layers (the state passes through trk_out between launches; in reference
+// semantics every launch restarts from trk_in, as every layer does).
+void launch_uniform(int sem, bool exact, int threads, int per_launch, hipStream_t st, const UniformLaunch& a) {
+  const dim3 grid(grid_for_n(a.n, threads)), block(threads);
+  const int step = (per_launch > 0 && per_launch < a.nlayer) ? per_launch : a.nlayer;
+  for (int l0 = 0; l0 < a.nlayer; l0 += step) {
+    const int l1 = std::min(a.nlayer, l0 + step);
+    const float* in = (sem == P2R_CHAINED && l0 > 0) ? a.O : a.T;
+    const bool acc = l0 > 0;
+    if (sem == P2R_REFERENCE) {
+      if (a.chi2) launch_uniform_p<P2R_REFERENCE, true>(exact, grid, block, st, a, in, l0, l1, acc);
+      else launch_uniform_p<P2R_REFERENCE, false>(exact, grid, block, st, a, in, l0, l1, acc);
+    } else {
+      if (a.chi2) launch_uniform_p<P2R_CHAINED, true>(exact, grid, block, st, a, in, l0, l1, acc);
+      else launch_uniform_p<P2R_CHAINED, false>(exact, grid, block, st, a, in, l0, l1, acc);
+    }
   }
 }
 
@@ -566,8 +591,9 @@ template <int SEM, bool CHI2, bool EXACT>
 int launch_ragged_r(int rounds, p2r_ctx* c, hipStream_t st, const float* T, const float* H, float* O, float* chi2,
                     const uint32_t* M, uint32_t nslot, uint32_t bs, int nlayer, float kf) {
   if (rounds == P2R_RAGGED_LAYER_ORDER) {
-    const uint32_t grid = (nslot + kThreads - 1) / kThreads;
-    fwd_ragged_layer_kernel<SEM, CHI2, EXACT><<<grid, kThreads, 0, st>>>(T, H, O, chi2, M, nslot, bs, nlayer, kf);
+    const int threads = c->tune.ragged_threads;
+    fwd_ragged_layer_kernel<SEM, CHI2, EXACT><<<grid_for_n(nslot, threads), threads, 0, st>>>(T, H, O, chi2, M, nslot,
+                                                                                            bs, nlayer, kf);
     return P2R_OK;
   }
   switch (rounds) {
@@ -680,17 +706,12 @@ int p2r_forward(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out,
   const p2r_params p = params ? *params : p2r_default_params();
   if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
   if (ntrk == 0) return P2R_OK;
-  const float kf = p.kfact / p.b_scale;
-  DeviceGuard g(c->device);
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  const dim3 grid(grid_for(ntrk)), block(kThreads);
-  auto T = static_cast<const float*>(trk_in);
-  auto H = static_cast<const float*>(hits);
-  auto O = static_cast<float*>(trk_out);
-  const uint32_t n = (uint32_t)ntrk, bs = (uint32_t)s->bsize;
   if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
-  launch_uniform(semantics, chi2 != nullptr, p.precision == P2R_EXACT, grid, block, st, T, H, O, chi2, n, bs,
-                 s->nlayer, kf);
+  DeviceGuard g(c->device);
+  const UniformLaunch a{static_cast<const float*>(trk_in), static_cast<const float*>(hits), static_cast<float*>(trk_out),
+                        chi2, (uint32_t)ntrk, (uint32_t)s->bsize, s->nlayer, p.kfact / p.b_scale};
+  launch_uniform(semantics, p.precision == P2R_EXACT, c->tune.fwd_threads, c->tune.fwd_layers_per_launch,
+                 static_cast<hipStream_t>(stream), a);
   return c->check_launch("p2r_forward");
 }
 
@@ -716,7 +737,7 @@ int p2r_forward_ragged(p2r_ctx* c, const void* trk_in, const void* hits, void* t
   if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
   if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
   if (nslot == 0) return P2R_OK;
-  const int rounds = tile_rounds != P2R_RAGGED_AUTO ? tile_rounds : P2R_RAGGED_LAYER_ORDER;
+  const int rounds = tile_rounds != P2R_RAGGED_AUTO ? tile_rounds : c->tune.ragged_schedule;
   DeviceGuard g(c->device);
   hipStream_t st = static_cast<hipStream_t>(stream);
   auto T = static_cast<const float*>(trk_in);
@@ -745,7 +766,7 @@ int p2r_forward_adjoint(p2r_ctx* c, const void* trk_in, const void* hits, void* 
   if (s->ntrks % s->bsize != 0) return c->fail(P2R_ERR_SHAPE, "ntrks must be a multiple of bsize");
   if (grad_precision < P2R_GRAD_F32 || grad_precision > P2R_GRAD_F64)
     return c->fail(P2R_ERR_ARG, "bad grad_precision %d", grad_precision);
-  const int cs = checkpoint_stride ? checkpoint_stride : 4;
+  const int cs = checkpoint_stride ? checkpoint_stride : c->tune.checkpoint_stride;
   if (cs != 1 && cs != 2 && cs != 4 && cs != 8)
     return c->fail(P2R_ERR_ARG, "checkpoint_stride must be 0, 1, 2, 4 or 8");
   const uint64_t ntrk = (uint64_t)s->nevts * (uint64_t)s->ntrks;

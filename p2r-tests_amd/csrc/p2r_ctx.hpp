@@ -24,6 +24,8 @@ struct p2r_ctx {
   int device = 0;
   int num_cus = 0;
   char msg[256] = "no error";
+  // Launch choices (p2r_set_tuning / the autotuners); defaults below.
+  p2r_tuning tune = {256, 0, P2R_RAGGED_LAYER_ORDER, 256, 4};
   // Workspace owned by the context (device memory), grown on demand.
   void* ws = nullptr;
   size_t ws_bytes = 0;

@@ -1,0 +1,273 @@
+// Launch choices of a context and the autotuners over them (include/p2r.h).
+//
+// The autotuners are host code over the public entry points: they set a
+// candidate tuning, time the entry point with HIP events on the caller's
+// stream, and keep the fastest.  The only kernel here is the Matriplex width
+// repack used to time other widths.
+#include "../../include/p2r.h"
+#include "p2r_ctx.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <functional>
+
+namespace {
+
+bool threads_ok(int t) { return t == 64 || t == 128 || t == 256; }
+bool schedule_ok(int s) { return s == P2R_RAGGED_LAYER_ORDER || s == 1 || s == 2 || s == 4; }
+bool stride_ok(int s) { return s == 1 || s == 2 || s == 4 || s == 8; }
+
+// Mean time (ms) of `reps` calls after 2 untimed ones; P2R status of the first failing call otherwise.
+int time_calls(p2r_ctx* c, hipStream_t st, int reps, const std::function<int()>& call, float* ms) {
+  for (int i = 0; i < 2; ++i) {
+    const int rc = call();
+    if (rc != P2R_OK) return rc;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return c->fail(P2R_ERR_HIP, "hipEventCreate failed");
+  }
+  int rc = P2R_OK;
+  if (hipEventRecord(e0, st) != hipSuccess) rc = c->fail(P2R_ERR_HIP, "hipEventRecord failed");
+  for (int i = 0; rc == P2R_OK && i < reps; ++i) rc = call();
+  if (rc == P2R_OK && hipEventRecord(e1, st) != hipSuccess) rc = c->fail(P2R_ERR_HIP, "hipEventRecord failed");
+  if (rc == P2R_OK && hipEventSynchronize(e1) != hipSuccess) rc = c->fail(P2R_ERR_HIP, "hipEventSynchronize failed");
+  float t = 0.f;
+  if (rc == P2R_OK && hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+    rc = c->fail(P2R_ERR_HIP, "hipEventElapsedTime failed");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc == P2R_OK) *ms = t / (float)reps;
+  return rc;
+}
+
+struct Collect {
+  p2r_tune_report rep{};
+  void add(const p2r_tune_entry& e) {
+    if (rep.count < 32) rep.entry[rep.count++] = e;
+  }
+  void finish(p2r_tune_report* out) {
+    rep.best = 0;
+    for (int i = 1; i < rep.count; ++i)
+      if (rep.entry[i].ms < rep.entry[rep.best].ms) rep.best = i;
+    if (out) *out = rep;
+  }
+};
+
+p2r_tune_entry entry_of(const p2r_tuning& t, int bsize, float ms, int schedule) {
+  return p2r_tune_entry{bsize, schedule == P2R_RAGGED_LAYER_ORDER ? t.ragged_threads : t.fwd_threads,
+                        t.fwd_layers_per_launch, schedule, t.checkpoint_stride, ms};
+}
+
+// dst (dbs layout) <- src (sbs layout), one lane-word per thread; consecutive
+// threads take consecutive tracks of one (event, sub-block, word), so reads and
+// writes run along Matriplex rows.
+__global__ void repack_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, uint32_t sbs,
+                              uint32_t dbs, uint32_t ntrks, uint32_t nsub, uint32_t words, uint64_t total) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    const uint64_t t = i % ntrks;
+    uint64_t r = i / ntrks;
+    const uint64_t w = r % words;
+    r /= words;
+    const uint64_t sub = r % nsub;
+    const uint64_t e = r / nsub;
+    const uint64_t sblk = e * (ntrks / sbs) + t / sbs, dblk = e * (ntrks / dbs) + t / dbs;
+    dst[((dblk * nsub + sub) * words + w) * dbs + t % dbs] = src[((sblk * nsub + sub) * words + w) * sbs + t % sbs];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int p2r_get_tuning(const p2r_ctx* c, p2r_tuning* out) {
+  if (!c || !out) return P2R_ERR_ARG;
+  *out = c->tune;
+  return P2R_OK;
+}
+
+int p2r_set_tuning(p2r_ctx* c, const p2r_tuning* in) {
+  if (!c) return P2R_ERR_ARG;
+  if (!in) return c->fail(P2R_ERR_ARG, "null tuning");
+  if (!threads_ok(in->fwd_threads) || !threads_ok(in->ragged_threads))
+    return c->fail(P2R_ERR_ARG, "threads must be 64, 128 or 256");
+  if (in->fwd_layers_per_launch < 0) return c->fail(P2R_ERR_ARG, "fwd_layers_per_launch must be >= 0");
+  if (!schedule_ok(in->ragged_schedule))
+    return c->fail(P2R_ERR_ARG, "ragged_schedule must be P2R_RAGGED_LAYER_ORDER, 1, 2 or 4");
+  if (!stride_ok(in->checkpoint_stride)) return c->fail(P2R_ERR_ARG, "checkpoint_stride must be 1, 2, 4 or 8");
+  c->tune = *in;
+  return P2R_OK;
+}
+
+int p2r_autotune_forward(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* s,
+                         int semantics, const p2r_params* params, int32_t reps, p2r_tune_report* report,
+                         void* stream) {
+  if (!c) return P2R_ERR_ARG;
+  if (!s || reps < 1) return c->fail(P2R_ERR_ARG, "null shape or reps < 1");
+  const p2r_tuning keep = c->tune;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Collect col;
+  const int per[3] = {0, (s->nlayer + 1) / 2, (s->nlayer + 3) / 4};
+  for (int pi = 0; pi < 3; ++pi) {
+    if (pi > 0 && (per[pi] >= s->nlayer || per[pi] == per[pi - 1])) continue;
+    for (int threads : {64, 128, 256}) {
+      c->tune = keep;
+      c->tune.fwd_threads = threads;
+      c->tune.fwd_layers_per_launch = per[pi];
+      float ms = 0.f;
+      const int rc = time_calls(
+          c, st, reps, [&] { return p2r_forward(c, trk_in, hits, trk_out, s, semantics, params, nullptr, stream); },
+          &ms);
+      if (rc != P2R_OK) {
+        c->tune = keep;
+        return rc;
+      }
+      col.add(entry_of(c->tune, s->bsize, ms, 0));
+    }
+  }
+  col.finish(report);
+  const p2r_tune_entry& b = col.rep.entry[col.rep.best];
+  c->tune = keep;
+  c->tune.fwd_threads = b.threads;
+  c->tune.fwd_layers_per_launch = b.layers_per_launch;
+  return P2R_OK;
+}
+
+int p2r_autotune_ragged(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out, const int64_t* blk_offsets,
+                        int32_t nevts, const uint32_t* layer_mask, int32_t nlayer, int32_t bsize, int semantics,
+                        const p2r_params* params, int32_t reps, p2r_tune_report* report, void* stream) {
+  if (!c) return P2R_ERR_ARG;
+  if (reps < 1) return c->fail(P2R_ERR_ARG, "reps < 1");
+  const p2r_tuning keep = c->tune;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Collect col;
+  struct Cand {
+    int schedule, threads;
+  };
+  const Cand cands[] = {{P2R_RAGGED_LAYER_ORDER, 64}, {P2R_RAGGED_LAYER_ORDER, 128}, {P2R_RAGGED_LAYER_ORDER, 256},
+                        {1, 256}, {2, 256}, {4, 256}};
+  for (const Cand& cd : cands) {
+    c->tune = keep;
+    c->tune.ragged_schedule = cd.schedule;
+    c->tune.ragged_threads = cd.threads;
+    float ms = 0.f;
+    const int rc = time_calls(
+        c, st, reps,
+        [&] {
+          return p2r_forward_ragged(c, trk_in, hits, trk_out, blk_offsets, nevts, layer_mask, nlayer, bsize,
+                                    semantics, params, nullptr, P2R_RAGGED_AUTO, stream);
+        },
+        &ms);
+    if (rc != P2R_OK) {
+      c->tune = keep;
+      return rc;
+    }
+    p2r_tune_entry e = entry_of(c->tune, bsize, ms, cd.schedule);
+    e.threads = cd.schedule == P2R_RAGGED_LAYER_ORDER ? cd.threads : 256;
+    col.add(e);
+  }
+  col.finish(report);
+  const p2r_tune_entry& b = col.rep.entry[col.rep.best];
+  c->tune = keep;
+  c->tune.ragged_schedule = b.schedule;
+  if (b.schedule == P2R_RAGGED_LAYER_ORDER) c->tune.ragged_threads = b.threads;
+  return P2R_OK;
+}
+
+int p2r_autotune_adjoint(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* s,
+                         const p2r_params* params, int grad_precision, float* chi2, float* grad_par,
+                         float* grad_hits, float* grad_kfact, double* grad_global, int32_t reps,
+                         p2r_tune_report* report, void* stream) {
+  if (!c) return P2R_ERR_ARG;
+  if (!s || reps < 1) return c->fail(P2R_ERR_ARG, "null shape or reps < 1");
+  const p2r_tuning keep = c->tune;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Collect col;
+  for (int cs : {1, 2, 4, 8}) {
+    float ms = 0.f;
+    const int rc = time_calls(
+        c, st, reps,
+        [&] {
+          return p2r_forward_adjoint(c, trk_in, hits, trk_out, s, params, grad_precision, cs, chi2, grad_par,
+                                     grad_hits, grad_kfact, grad_global, stream);
+        },
+        &ms);
+    if (rc != P2R_OK) {
+      c->tune = keep;
+      return rc;
+    }
+    p2r_tuning t = keep;
+    t.checkpoint_stride = cs;
+    col.add(entry_of(t, s->bsize, ms, 0));
+  }
+  col.finish(report);
+  c->tune = keep;
+  c->tune.checkpoint_stride = col.rep.entry[col.rep.best].checkpoint_stride;
+  return P2R_OK;
+}
+
+int p2r_repack(p2r_ctx* c, const void* src, int32_t sbs, void* dst, int32_t dbs, int32_t nevts, int32_t ntrks,
+               int32_t nsub, int32_t words, void* stream) {
+  if (!c) return P2R_ERR_ARG;
+  if (!src || !dst) return c->fail(P2R_ERR_ARG, "null pointer argument");
+  if (sbs < 1 || sbs > 1024 || dbs < 1 || dbs > 1024 || nevts < 0 || ntrks < 0 || nsub < 1 || words < 1 ||
+      ntrks % sbs != 0 || ntrks % dbs != 0)
+    return c->fail(P2R_ERR_SHAPE, "bad repack shape (widths must divide ntrks)");
+  const uint64_t total = (uint64_t)nevts * (uint64_t)ntrks * (uint64_t)nsub * (uint64_t)words;
+  if (total == 0) return P2R_OK;
+  const uint64_t bytes = total * 4;
+  const char *a = static_cast<const char*>(src), *b = static_cast<const char*>(dst);
+  if (a < b + bytes && b < a + bytes) return c->fail(P2R_ERR_ARG, "src and dst overlap");
+  DeviceGuard g(c->device);
+  const unsigned grid = (unsigned)std::min<uint64_t>((total + 255) / 256, (uint64_t)c->num_cus * 32);
+  repack_kernel<<<grid, 256, 0, static_cast<hipStream_t>(stream)>>>(
+      static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), (uint32_t)sbs, (uint32_t)dbs, (uint32_t)ntrks,
+      (uint32_t)nsub, (uint32_t)words, total);
+  return c->check_launch("p2r_repack");
+}
+
+int p2r_autotune_width(p2r_ctx* c, const void* trk_in, const void* hits, const p2r_shape* s, int semantics,
+                       const p2r_params* params, const int32_t* widths, int32_t nwidths, int32_t reps,
+                       p2r_tune_report* report, void* stream) {
+  if (!c) return P2R_ERR_ARG;
+  if (!s || !widths || nwidths < 1 || nwidths > 32 || reps < 1)
+    return c->fail(P2R_ERR_ARG, "null shape/widths, nwidths outside 1..32 or reps < 1");
+  for (int i = 0; i < nwidths; ++i)
+    if (widths[i] < 1 || widths[i] > 1024 || s->ntrks % widths[i] != 0)
+      return c->fail(P2R_ERR_SHAPE, "width %d does not divide ntrks=%d", widths[i], s->ntrks);
+  DeviceGuard g(c->device);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const size_t ntrk = (size_t)s->nevts * (size_t)s->ntrks;
+  const size_t tb = ntrk * 28 * 4, hb = ntrk * (size_t)s->nlayer * 9 * 4;
+  void *t = nullptr, *h = nullptr, *o = nullptr;
+  if (hipMalloc(&t, tb) != hipSuccess || hipMalloc(&h, hb) != hipSuccess || hipMalloc(&o, tb) != hipSuccess) {
+    if (t) (void)hipFree(t);
+    if (h) (void)hipFree(h);
+    return c->fail(P2R_ERR_NOMEM, "autotune_width copies (%zu bytes)", 2 * tb + hb);
+  }
+  Collect col;
+  int rc = P2R_OK;
+  for (int i = 0; rc == P2R_OK && i < nwidths; ++i) {
+    const int w = widths[i];
+    rc = p2r_repack(c, trk_in, s->bsize, t, w, s->nevts, s->ntrks, 1, 28, stream);
+    if (rc == P2R_OK) rc = p2r_repack(c, hits, s->bsize, h, w, s->nevts, s->ntrks, s->nlayer, 9, stream);
+    const p2r_shape sw{s->nevts, s->ntrks, s->nlayer, w};
+    float ms = 0.f;
+    if (rc == P2R_OK)
+      rc = time_calls(c, st, reps, [&] { return p2r_forward(c, t, h, o, &sw, semantics, params, nullptr, stream); },
+                      &ms);
+    if (rc == P2R_OK) col.add(entry_of(c->tune, w, ms, 0));
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(t);
+  (void)hipFree(h);
+  (void)hipFree(o);
+  if (rc != P2R_OK) return rc;
+  col.finish(report);
+  return P2R_OK;
+}
+
+}  // extern "C"

@@ -12,7 +12,8 @@ import ctypes as C
 import numpy as np
 
 from ._capi import (P2R_CHAINED, P2R_EXACT, P2R_FAST, P2R_GRAD_F32, P2R_GRAD_MIXED, P2R_GRAD_F64, P2R_RAGGED_AUTO, P2R_RAGGED_LAYER_ORDER, P2R_ERR_ARG, P2R_ERR_HIP, P2R_ERR_NOMEM, P2R_ERR_SHAPE, P2R_ERR_UNSUPPORTED,
-                    P2R_OK, P2R_REFERENCE, P2RError, lib, p2r_gen_config, p2r_params, p2r_shape)
+                    P2R_OK, P2R_REFERENCE, P2RError, lib, p2r_gen_config, p2r_params, p2r_shape, p2r_tune_report,
+                    p2r_tuning)
 from .matriplex import Shape, tracks_to_arrays, hits_block_view  # noqa: F401
 
 SEMANTICS = {"reference": P2R_REFERENCE, "chained": P2R_CHAINED}
@@ -103,6 +104,70 @@ class Context:
                                        int(checkpoint_stride), _ptr(chi2), _ptr(grad_par), _ptr(grad_hits),
                                        _ptr(grad_kfact), _ptr(grad_global), _stream(stream))
         self._check(st, "p2r_forward_adjoint")
+
+    # ---- launch choices and autotuners (include/p2r.h p2r_tuning, p2r_autotune_*) ----
+    def get_tuning(self):
+        t = p2r_tuning()
+        self._check(lib().p2r_get_tuning(self._h, C.byref(t)), "p2r_get_tuning")
+        return t
+
+    def set_tuning(self, t=None, **kw):
+        t = t if t is not None else self.get_tuning()
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        self._check(lib().p2r_set_tuning(self._h, C.byref(t)), "p2r_set_tuning")
+        return t
+
+    def autotune_forward(self, trk, hits, out, shape, semantics="reference", params=None, precision=None, reps=5,
+                         stream=None):
+        """Times fwd_threads x fwd_layers_per_launch on these buffers, keeps the fastest; returns the report."""
+        sem = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
+        sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
+        p = params if params is not None else default_params(precision or "exact")
+        r = p2r_tune_report()
+        self._check(lib().p2r_autotune_forward(self._h, _ptr(trk), _ptr(hits), _ptr(out), C.byref(sh), sem,
+                                               C.byref(p), int(reps), C.byref(r), _stream(stream)),
+                    "p2r_autotune_forward")
+        return r
+
+    def autotune_ragged(self, trk, hits, out, blk_offsets, layer_mask, nlayer, bsize, semantics="chained",
+                        params=None, precision=None, reps=5, stream=None):
+        sem = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
+        p = params if params is not None else default_params(precision or "exact")
+        offs = np.ascontiguousarray(blk_offsets, np.int64)
+        r = p2r_tune_report()
+        self._check(lib().p2r_autotune_ragged(self._h, _ptr(trk), _ptr(hits), _ptr(out), _ptr(offs), len(offs) - 1,
+                                              _ptr(layer_mask), int(nlayer), int(bsize), sem, C.byref(p), int(reps),
+                                              C.byref(r), _stream(stream)), "p2r_autotune_ragged")
+        return r
+
+    def autotune_adjoint(self, trk, hits, out, shape, params=None, grad_precision="f32", chi2=None, grad_par=None,
+                         grad_hits=None, grad_kfact=None, grad_global=None, precision=None, reps=3, stream=None):
+        sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
+        p = params if params is not None else default_params(precision or "exact")
+        gp = GRAD_PRECISION[grad_precision] if isinstance(grad_precision, str) else int(grad_precision)
+        r = p2r_tune_report()
+        self._check(lib().p2r_autotune_adjoint(self._h, _ptr(trk), _ptr(hits), _ptr(out), C.byref(sh), C.byref(p), gp,
+                                               _ptr(chi2), _ptr(grad_par), _ptr(grad_hits), _ptr(grad_kfact),
+                                               _ptr(grad_global), int(reps), C.byref(r), _stream(stream)),
+                    "p2r_autotune_adjoint")
+        return r
+
+    def repack(self, src, src_bsize, dst, dst_bsize, nevts, ntrks, nsub, words, stream=None):
+        """Matriplex width conversion on the device (MPTRK: nsub=1, words=28; MPHIT: nsub=nlayer, words=9)."""
+        self._check(lib().p2r_repack(self._h, _ptr(src), int(src_bsize), _ptr(dst), int(dst_bsize), int(nevts),
+                                     int(ntrks), int(nsub), int(words), _stream(stream)), "p2r_repack")
+
+    def autotune_width(self, trk, hits, shape, widths=(16, 32, 64), semantics="reference", params=None,
+                       precision=None, reps=5, stream=None):
+        sem = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
+        sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
+        p = params if params is not None else default_params(precision or "exact")
+        w = np.ascontiguousarray(widths, np.int32)
+        r = p2r_tune_report()
+        self._check(lib().p2r_autotune_width(self._h, _ptr(trk), _ptr(hits), C.byref(sh), sem, C.byref(p), _ptr(w),
+                                             len(w), int(reps), C.byref(r), _stream(stream)), "p2r_autotune_width")
+        return r
 
     def propagateToR(self, inErr, inPar, inChg, msP, outErr, outPar, bsize, nblk, trk_stride, hit_stride,
                      kfact=None, stream=None):

@@ -28,6 +28,26 @@ class p2r_gen_config(C.Structure):
                 ("hit_var", C.c_float), ("layer_offsets", C.c_void_p), ("b_scale_truth", C.c_float)]
 
 
+class p2r_tuning(C.Structure):
+    _fields_ = [("fwd_threads", C.c_int32), ("fwd_layers_per_launch", C.c_int32), ("ragged_schedule", C.c_int32),
+                ("ragged_threads", C.c_int32), ("checkpoint_stride", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class p2r_tune_entry(C.Structure):
+    _fields_ = [("bsize", C.c_int32), ("threads", C.c_int32), ("layers_per_launch", C.c_int32),
+                ("schedule", C.c_int32), ("checkpoint_stride", C.c_int32), ("ms", C.c_float)]
+
+
+class p2r_tune_report(C.Structure):
+    _fields_ = [("count", C.c_int32), ("best", C.c_int32), ("entry", p2r_tune_entry * 32)]
+
+    def as_list(self):
+        return [{k: getattr(self.entry[i], k) for k, _ in p2r_tune_entry._fields_} for i in range(self.count)]
+
+
 _lib = None
 
 
@@ -60,6 +80,17 @@ def lib():
             "p2r_gen_ragged_layout": ([vp, i32, i32, vp, vp], C.c_int),
             "p2r_gen_ragged": ([vp, vp, vp, vp, i32, vp, i32, i32, i32, i32, C.POINTER(p2r_gen_config)], C.c_int),
             "p2r_gen_lognormal_counts": ([vp, i32, C.c_double, C.c_double, i32, i32, C.c_uint64], C.c_int),
+            "p2r_get_tuning": ([vp, C.POINTER(p2r_tuning)], C.c_int),
+            "p2r_set_tuning": ([vp, C.POINTER(p2r_tuning)], C.c_int),
+            "p2r_autotune_forward": ([vp, vp, vp, vp, C.POINTER(p2r_shape), C.c_int, C.POINTER(p2r_params), i32,
+                                      C.POINTER(p2r_tune_report), vp], C.c_int),
+            "p2r_autotune_ragged": ([vp, vp, vp, vp, vp, i32, vp, i32, i32, C.c_int, C.POINTER(p2r_params), i32,
+                                     C.POINTER(p2r_tune_report), vp], C.c_int),
+            "p2r_autotune_adjoint": ([vp, vp, vp, vp, C.POINTER(p2r_shape), C.POINTER(p2r_params), C.c_int, vp, vp,
+                                      vp, vp, vp, i32, C.POINTER(p2r_tune_report), vp], C.c_int),
+            "p2r_repack": ([vp, vp, i32, vp, i32, i32, i32, i32, i32, vp], C.c_int),
+            "p2r_autotune_width": ([vp, vp, vp, C.POINTER(p2r_shape), C.c_int, C.POINTER(p2r_params), vp, i32, i32,
+                                    C.POINTER(p2r_tune_report), vp], C.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)

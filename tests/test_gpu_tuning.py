@@ -1,0 +1,145 @@
+"""GPU tests of the launch choices (p2r_set_tuning) and the autotuners: every choice gives bit-identical
+results (the per-lane arithmetic does not depend on workgroup size, layer chunking or Matriplex width),
+the autotuners pick one of their candidates and leave the context set to it, and the width repack is a
+permutation that round-trips."""
+import numpy as np
+import pytest
+
+from p2r_amd import Shape
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture()
+def ctx():
+    import p2r_amd
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = p2r_amd.Context(0)
+    yield c
+    c.close()
+
+
+def _fwd(ctx, dt, dh, shape, sem, precision="exact"):
+    do = torch.empty_like(dt)
+    dc = torch.empty(shape.ntrk_total, device="cuda")
+    ctx.forward(dt, dh, do, shape, sem, chi2=dc, precision=precision)
+    torch.cuda.synchronize()
+    return do.cpu().numpy().view(np.uint32), dc.cpu().numpy().view(np.uint32)
+
+
+@pytest.fixture(scope="module")
+def batch():
+    import p2r_amd
+    shape = Shape(3, 1024, 20, 32)
+    trk, hits = p2r_amd.gen_uniform(shape)
+    return shape, torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
+
+
+def test_tuning_roundtrip_and_validation(ctx):
+    import p2r_amd
+    t = ctx.get_tuning()
+    assert t.as_dict() == {"fwd_threads": 256, "fwd_layers_per_launch": 0, "ragged_schedule": -1,
+                           "ragged_threads": 256, "checkpoint_stride": 4}
+    ctx.set_tuning(fwd_threads=128, checkpoint_stride=2)
+    assert ctx.get_tuning().fwd_threads == 128 and ctx.get_tuning().checkpoint_stride == 2
+    for bad in ({"fwd_threads": 96}, {"ragged_schedule": 3}, {"checkpoint_stride": 3}, {"fwd_layers_per_launch": -1}):
+        with pytest.raises(p2r_amd.P2RError):
+            ctx.set_tuning(**bad)
+    assert ctx.get_tuning().fwd_threads == 128  # a rejected setting changes nothing
+
+
+@pytest.mark.parametrize("sem", ["reference", "chained"])
+def test_launch_choices_are_bit_identical(ctx, batch, sem):
+    shape, dt, dh = batch
+    ref_o, ref_c = _fwd(ctx, dt, dh, shape, sem)
+    for threads in (64, 128):
+        for per in (0, 7, 1):
+            ctx.set_tuning(fwd_threads=threads, fwd_layers_per_launch=per)
+            o, c = _fwd(ctx, dt, dh, shape, sem)
+            assert np.array_equal(o, ref_o) and np.array_equal(c, ref_c), (threads, per)
+
+
+def test_autotune_forward_picks_a_candidate(ctx, batch):
+    shape, dt, dh = batch
+    do = torch.empty_like(dt)
+    rep = ctx.autotune_forward(dt, dh, do, shape, "reference", reps=2)
+    ents = rep.as_list()
+    assert rep.count == len(ents) >= 6 and 0 <= rep.best < rep.count
+    assert all(e["ms"] > 0 for e in ents)
+    best = ents[rep.best]
+    assert best["ms"] == min(e["ms"] for e in ents)
+    t = ctx.get_tuning()
+    assert (t.fwd_threads, t.fwd_layers_per_launch) == (best["threads"], best["layers_per_launch"])
+    o, c = _fwd(ctx, dt, dh, shape, "reference")
+    ctx.set_tuning(fwd_threads=256, fwd_layers_per_launch=0)
+    o0, c0 = _fwd(ctx, dt, dh, shape, "reference")
+    assert np.array_equal(o, o0) and np.array_equal(c, c0)
+
+
+def test_repack_round_trip_and_width_invariance(ctx, batch):
+    """32 -> 16 -> 32 restores every word; the forward on the width-16 copy equals the width-32 forward
+    after repacking its output back."""
+    shape, dt, dh = batch
+    t16, h16 = torch.empty_like(dt), torch.empty_like(dh)
+    ctx.repack(dt, 32, t16, 16, shape.nevts, shape.ntrks, 1, 28)
+    ctx.repack(dh, 32, h16, 16, shape.nevts, shape.ntrks, shape.nlayer, 9)
+    t32, h32 = torch.empty_like(dt), torch.empty_like(dh)
+    ctx.repack(t16, 16, t32, 32, shape.nevts, shape.ntrks, 1, 28)
+    ctx.repack(h16, 16, h32, 32, shape.nevts, shape.ntrks, shape.nlayer, 9)
+    torch.cuda.synchronize()
+    assert torch.equal(t32.view(torch.int32), dt.view(torch.int32))
+    assert torch.equal(h32.view(torch.int32), dh.view(torch.int32))
+    # the layout itself: track t of event e, field f
+    T, T16 = dt.cpu().numpy().reshape(-1, 28, 32), t16.cpu().numpy().reshape(-1, 28, 16)
+    for e, t, f in ((0, 0, 0), (1, 37, 5), (2, 1023, 27)):
+        a = T[e * 32 + t // 32, f, t % 32].view(np.uint32)
+        b = T16[e * 64 + t // 16, f, t % 16].view(np.uint32)
+        assert a == b
+    s16 = Shape(shape.nevts, shape.ntrks, shape.nlayer, 16)
+    for sem in ("reference", "chained"):
+        o32, c32 = _fwd(ctx, dt, dh, shape, sem)
+        o16 = torch.empty_like(dt)
+        c16 = torch.empty(shape.ntrk_total, device="cuda")
+        ctx.forward(t16, h16, o16, s16, sem, chi2=c16, precision="exact")
+        back = torch.empty_like(dt)
+        ctx.repack(o16, 16, back, 32, shape.nevts, shape.ntrks, 1, 28)
+        torch.cuda.synchronize()
+        assert np.array_equal(back.cpu().numpy().view(np.uint32), o32)
+        assert np.array_equal(c16.cpu().numpy().view(np.uint32), c32)  # chi2 is indexed by flat track
+
+
+def test_autotune_width_reports_each_width(ctx, batch):
+    shape, dt, dh = batch
+    rep = ctx.autotune_width(dt, dh, shape, widths=(16, 32, 64), reps=2)
+    ents = rep.as_list()
+    assert [e["bsize"] for e in ents] == [16, 32, 64] and all(e["ms"] > 0 for e in ents)
+
+
+def test_autotune_ragged_and_adjoint(ctx):
+    import p2r_amd
+    counts = p2r_amd.lognormal_counts(12, median=600, sigma=0.5, lo=30, hi=3000, seed=5)
+    trk, hits, mask, offs = p2r_amd.gen_ragged(counts, 20, 32, kmin=5, kmax=20)
+    dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
+    dm = torch.from_numpy(mask.view(np.int32)).cuda()
+    do = torch.empty_like(dt)
+    ctx.forward_ragged(dt, dh, do, offs, dm, 20, 32, "chained")
+    torch.cuda.synchronize()
+    ref = do.cpu().numpy().view(np.uint32).copy()
+    rep = ctx.autotune_ragged(dt, dh, do, offs, dm, 20, 32, "chained", reps=2)
+    ents = rep.as_list()
+    assert rep.count == 6 and ctx.get_tuning().ragged_schedule == ents[rep.best]["schedule"]
+    for threads in (64, 128):
+        ctx.set_tuning(ragged_schedule=-1, ragged_threads=threads)
+        ctx.forward_ragged(dt, dh, do, offs, dm, 20, 32, "chained")
+        torch.cuda.synchronize()
+        assert np.array_equal(do.cpu().numpy().view(np.uint32), ref), threads
+    shape = Shape(1, 2048, 20, 32)
+    trk, hits = p2r_amd.gen_uniform(shape)
+    dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
+    do = torch.empty_like(dt)
+    rep = ctx.autotune_adjoint(dt, dh, do, shape, grad_precision="f32", reps=1)
+    ents = rep.as_list()
+    assert [e["checkpoint_stride"] for e in ents] == [1, 2, 4, 8]
+    assert ctx.get_tuning().checkpoint_stride == ents[rep.best]["checkpoint_stride"]
