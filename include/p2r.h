@@ -171,6 +171,20 @@ int p2r_propagateToR(p2r_ctx* ctx, const float* inErr, const float* inPar, const
 int p2r_KalmanUpdate(p2r_ctx* ctx, float* trkErr, float* inPar, const float* hitErr, const float* msP,
                      int32_t bsize, int64_t nblk, int64_t trk_stride, int64_t hit_stride, void* stream);
 
+/* Host-to-host forward in the reference's streamed form (its default timing
+ * mode, propagate-tor-test_hip.cu:905-1013 with MEASURE_H2D/D2H_TRANSFER): the
+ * batch's Matriplex blocks are cut into `nstreams` contiguous chunks; each
+ * chunk's tracks and hits are copied host -> device, run through all layers
+ * and copied back on a stream of its own (created by the context on first use,
+ * default priority), so the copies of one chunk overlap the kernel of another.
+ * host_*: the batch in host memory, pinned (hipHostMalloc) for the copies to be
+ * asynchronous; dev_*: device buffers of the same sizes, the caller's.
+ * Returns when every chunk is back on the host.  nstreams in 1..16.
+ * The reference times this with a wall clock around the same three steps. */
+int p2r_forward_streamed(p2r_ctx* ctx, const void* host_trk_in, const void* host_hits, void* host_trk_out,
+                         void* dev_trk_in, void* dev_hits, void* dev_trk_out, const p2r_shape* shape, int semantics,
+                         const p2r_params* params, int32_t nstreams);
+
 /* ---- Launch choices and the autotuner ---------------------------------------
  * The reference fixes Matriplex width, threads per block and layers per kernel
  * at compile time (build.py:68-80 sweeps them by rebuilding; hip_local.cu:911

@@ -11,7 +11,8 @@
 //
 //   p2r_bench [--ntrks N] [--nevts N] [--nlayer N] [--bsize N] [--NITER N]
 //             [--NWARMUP N] [--semantics reference|chained]
-//             [--input reference|baseline] [--include-transfers] [--device D]
+//             [--input reference|baseline] [--include-transfers] [--num-streams N]
+//             [--device D]
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -46,7 +47,7 @@
 namespace {
 
 struct Opts {
-  int ntrks = 9600, nevts = 100, nlayer = 20, bsize = 32, niter = 5, nwarmup = 2, device = 0;
+  int ntrks = 9600, nevts = 100, nlayer = 20, bsize = 32, niter = 5, nwarmup = 2, device = 0, nstreams = 1;
   int semantics = P2R_REFERENCE;
   bool reference_input = true, transfers = false;
 };
@@ -72,6 +73,7 @@ Opts parse(int argc, char** argv) {
     else if (a == "--semantics") o.semantics = std::string(next()) == "chained" ? P2R_CHAINED : P2R_REFERENCE;
     else if (a == "--input") o.reference_input = std::string(next()) != "baseline";
     else if (a == "--include-transfers") o.transfers = true;
+    else if (a == "--num-streams") o.nstreams = std::atoi(next());
     else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       std::exit(1);
@@ -201,13 +203,13 @@ int main(int argc, char** argv) {
   std::printf("Size of struct struct MPHIT hit[] = %ld\n", (long)(hit_floats * 4));
 
   const p2r_params params = p2r_default_params();
+  // with transfers: the reference's streamed form (hip.cu:941-1013), chunks of blocks over num_streams streams
   auto one = [&]() {
-    if (o.transfers) {
-      CHECK_HIP(hipMemcpyAsync(d_trk, h_trk, trk_floats * 4, hipMemcpyHostToDevice, st));
-      CHECK_HIP(hipMemcpyAsync(d_hit, h_hit, hit_floats * 4, hipMemcpyHostToDevice, st));
-    }
-    CHECK_P2R(ctx, p2r_forward(ctx, d_trk, d_hit, d_out, &shape, o.semantics, &params, nullptr, st));
-    if (o.transfers) CHECK_HIP(hipMemcpyAsync(h_out, d_out, trk_floats * 4, hipMemcpyDeviceToHost, st));
+    if (o.transfers)
+      CHECK_P2R(ctx, p2r_forward_streamed(ctx, h_trk, h_hit, h_out, d_trk, d_hit, d_out, &shape, o.semantics, &params,
+                                          o.nstreams));
+    else
+      CHECK_P2R(ctx, p2r_forward(ctx, d_trk, d_hit, d_out, &shape, o.semantics, &params, nullptr, st));
   };
   for (int i = 0; i < o.nwarmup; ++i) one();
   CHECK_HIP(hipStreamSynchronize(st));

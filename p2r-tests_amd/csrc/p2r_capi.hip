@@ -64,7 +64,8 @@ __global__ __launch_bounds__(kThreads) void fwd_uniform_kernel(const float* trk,
   const float* hb = hits + (size_t)blk * nlayer * 9 * bs + lane;
   const size_t hstride = (size_t)9 * bs;
   Trk cur = in;
-  float c2 = 0.f;
+  // a later launch of a split layer loop continues the running sum (same rounding as one launch)
+  float c2 = (CHI2 && acc) ? chi2[g] : 0.f;
   Hit hn;
   load_hit(hb + (size_t)l0 * hstride, bs, hn);
   for (int l = l0; l < l1; ++l) {
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(kThreads) void fwd_uniform_kernel(const float* trk,
   float* ob = out + (size_t)blk * 28 * bs + lane;
   store_trk(ob, bs, cur);
   reinterpret_cast<int*>(ob)[(size_t)27 * bs] = q;
-  if (CHI2) chi2[g] = acc ? chi2[g] + c2 : c2;
+  if (CHI2) chi2[g] = c2;
 }
 
 
@@ -683,6 +684,7 @@ int p2r_destroy(p2r_ctx* c) {
   {
     DeviceGuard g(c->device);
     c->release_workspace();
+    c->release_streams();
   }
   delete c;
   return P2R_OK;
@@ -713,6 +715,55 @@ int p2r_forward(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out,
   launch_uniform(semantics, p.precision == P2R_EXACT, c->tune.fwd_threads, c->tune.fwd_layers_per_launch,
                  static_cast<hipStream_t>(stream), a);
   return c->check_launch("p2r_forward");
+}
+
+
+int p2r_forward_streamed(p2r_ctx* c, const void* host_trk_in, const void* host_hits, void* host_trk_out,
+                         void* dev_trk_in, void* dev_hits, void* dev_trk_out, const p2r_shape* s, int semantics,
+                         const p2r_params* params, int32_t nstreams) {
+  if (!c) return P2R_ERR_ARG;
+  if (!host_trk_in || !host_hits || !host_trk_out || !dev_trk_in || !dev_hits || !dev_trk_out || !s)
+    return c->fail(P2R_ERR_ARG, "null pointer argument");
+  if (semantics != P2R_REFERENCE && semantics != P2R_CHAINED) return c->fail(P2R_ERR_ARG, "bad semantics %d", semantics);
+  if (nstreams < 1 || nstreams > 16) return c->fail(P2R_ERR_ARG, "nstreams must be in 1..16");
+  if (s->nevts < 0 || s->ntrks < 0 || s->nlayer < 1 || s->bsize < 1 || s->bsize > 1024 || s->ntrks % s->bsize != 0)
+    return c->fail(P2R_ERR_SHAPE, "bad shape nevts=%d ntrks=%d nlayer=%d bsize=%d", s->nevts, s->ntrks, s->nlayer,
+                   s->bsize);
+  const uint64_t nblk = (uint64_t)s->nevts * (uint64_t)(s->ntrks / s->bsize);
+  if (nblk * (uint64_t)s->bsize >= (1ull << 31)) return c->fail(P2R_ERR_SHAPE, "too many tracks");
+  const p2r_params p = params ? *params : p2r_default_params();
+  if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
+  if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
+  if (nblk == 0) return P2R_OK;
+  DeviceGuard g(c->device);
+  const size_t bs = (size_t)s->bsize, tw = 28 * bs * 4, hw = (size_t)s->nlayer * 9 * bs * 4;  // bytes per block
+  const int ns = (int)std::min<uint64_t>((uint64_t)nstreams, nblk);
+  for (int k = 0; k < ns; ++k) {
+    hipStream_t st = c->stream(k);
+    if (!st) return c->fail(P2R_ERR_HIP, "hipStreamCreateWithFlags failed");
+    const uint64_t b0 = nblk * k / ns, b1 = nblk * (k + 1) / ns;  // contiguous block range of this chunk
+    if (b1 == b0) continue;
+    const size_t nb = b1 - b0;
+    auto off = [](const void* base, size_t bytes) { return static_cast<const char*>(base) + bytes; };
+    char* dt = static_cast<char*>(dev_trk_in) + b0 * tw;
+    char* dh = static_cast<char*>(dev_hits) + b0 * hw;
+    char* dout = static_cast<char*>(dev_trk_out) + b0 * tw;
+    if (hipMemcpyAsync(dt, off(host_trk_in, b0 * tw), nb * tw, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(dh, off(host_hits, b0 * hw), nb * hw, hipMemcpyHostToDevice, st) != hipSuccess)
+      return c->fail(P2R_ERR_HIP, "host-to-device copy of chunk %d failed", k);
+    const UniformLaunch a{reinterpret_cast<const float*>(dt), reinterpret_cast<const float*>(dh),
+                          reinterpret_cast<float*>(dout), nullptr, (uint32_t)(nb * bs), (uint32_t)bs, s->nlayer,
+                          p.kfact / p.b_scale};
+    launch_uniform(semantics, p.precision == P2R_EXACT, c->tune.fwd_threads, c->tune.fwd_layers_per_launch, st, a);
+    const int rc = c->check_launch("p2r_forward_streamed");
+    if (rc != P2R_OK) return rc;
+    if (hipMemcpyAsync(static_cast<char*>(host_trk_out) + b0 * tw, dout, nb * tw, hipMemcpyDeviceToHost, st) !=
+        hipSuccess)
+      return c->fail(P2R_ERR_HIP, "device-to-host copy of chunk %d failed", k);
+  }
+  for (int k = 0; k < ns; ++k)
+    if (hipStreamSynchronize(c->stream(k)) != hipSuccess) return c->fail(P2R_ERR_HIP, "stream %d failed", k);
+  return P2R_OK;
 }
 
 

@@ -26,6 +26,19 @@ struct p2r_ctx {
   char msg[256] = "no error";
   // Launch choices (p2r_set_tuning / the autotuners); defaults below.
   p2r_tuning tune = {256, 0, P2R_RAGGED_LAYER_ORDER, 256, 4};
+  // Streams of p2r_forward_streamed, created on first use (default priority).
+  hipStream_t streams[16] = {};
+  hipStream_t stream(int i) {
+    if (!streams[i] && hipStreamCreateWithFlags(&streams[i], hipStreamNonBlocking) != hipSuccess) streams[i] = nullptr;
+    return streams[i];
+  }
+  void release_streams() {
+    for (hipStream_t& s : streams)
+      if (s) {
+        (void)hipStreamDestroy(s);
+        s = nullptr;
+      }
+  }
   // Workspace owned by the context (device memory), grown on demand.
   void* ws = nullptr;
   size_t ws_bytes = 0;

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <vector>
 
 namespace {
 
@@ -40,6 +41,35 @@ int time_calls(p2r_ctx* c, hipStream_t st, int reps, const std::function<int()>&
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc == P2R_OK) *ms = t / (float)reps;
+  return rc;
+}
+
+// Runs `call` until >= 100 ms of GPU time have passed (at most 200 rounds of 12 calls): the clocks ramp up in the
+// first tens of milliseconds of work, which would otherwise favour the candidates timed last.
+int warm_up(p2r_ctx* c, hipStream_t st, const std::function<int()>& call) {
+  float total = 0.f;
+  for (int round = 0; round < 200 && total < 100.f; ++round) {
+    float ms = 0.f;
+    const int rc = time_calls(c, st, 10, call, &ms);
+    if (rc != P2R_OK) return rc;
+    total += 12.f * ms;
+  }
+  return P2R_OK;
+}
+
+// Times candidate i (selected by set(i)) for i in [0, n) in two interleaved passes and keeps the smaller
+// mean of each; set(i) must leave the context ready for call().
+int time_candidates(p2r_ctx* c, hipStream_t st, int reps, int n, const std::function<void(int)>& set,
+                    const std::function<int()>& call, float* best_ms) {
+  set(0);
+  int rc = warm_up(c, st, call);
+  for (int pass = 0; rc == P2R_OK && pass < 2; ++pass)
+    for (int i = 0; rc == P2R_OK && i < n; ++i) {
+      set(i);
+      float ms = 0.f;
+      rc = time_calls(c, st, reps, call, &ms);
+      if (rc == P2R_OK && (pass == 0 || ms < best_ms[i])) best_ms[i] = ms;
+    }
   return rc;
 }
 
@@ -109,28 +139,27 @@ int p2r_autotune_forward(p2r_ctx* c, const void* trk_in, const void* hits, void*
   if (!s || reps < 1) return c->fail(P2R_ERR_ARG, "null shape or reps < 1");
   const p2r_tuning keep = c->tune;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  Collect col;
+  std::vector<p2r_tuning> cand;
   const int per[3] = {0, (s->nlayer + 1) / 2, (s->nlayer + 3) / 4};
   for (int pi = 0; pi < 3; ++pi) {
     if (pi > 0 && (per[pi] >= s->nlayer || per[pi] == per[pi - 1])) continue;
     for (int threads : {64, 128, 256}) {
-      c->tune = keep;
-      c->tune.fwd_threads = threads;
-      c->tune.fwd_layers_per_launch = per[pi];
-      float ms = 0.f;
-      const int rc = time_calls(
-          c, st, reps, [&] { return p2r_forward(c, trk_in, hits, trk_out, s, semantics, params, nullptr, stream); },
-          &ms);
-      if (rc != P2R_OK) {
-        c->tune = keep;
-        return rc;
-      }
-      col.add(entry_of(c->tune, s->bsize, ms, 0));
+      p2r_tuning t = keep;
+      t.fwd_threads = threads;
+      t.fwd_layers_per_launch = per[pi];
+      cand.push_back(t);
     }
   }
+  std::vector<float> ms(cand.size(), 0.f);
+  const int rc = time_candidates(
+      c, st, reps, (int)cand.size(), [&](int i) { c->tune = cand[i]; },
+      [&] { return p2r_forward(c, trk_in, hits, trk_out, s, semantics, params, nullptr, stream); }, ms.data());
+  c->tune = keep;
+  if (rc != P2R_OK) return rc;
+  Collect col;
+  for (size_t i = 0; i < cand.size(); ++i) col.add(entry_of(cand[i], s->bsize, ms[i], 0));
   col.finish(report);
   const p2r_tune_entry& b = col.rep.entry[col.rep.best];
-  c->tune = keep;
   c->tune.fwd_threads = b.threads;
   c->tune.fwd_layers_per_launch = b.layers_per_launch;
   return P2R_OK;
@@ -143,35 +172,35 @@ int p2r_autotune_ragged(p2r_ctx* c, const void* trk_in, const void* hits, void* 
   if (reps < 1) return c->fail(P2R_ERR_ARG, "reps < 1");
   const p2r_tuning keep = c->tune;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  Collect col;
   struct Cand {
     int schedule, threads;
   };
-  const Cand cands[] = {{P2R_RAGGED_LAYER_ORDER, 64}, {P2R_RAGGED_LAYER_ORDER, 128}, {P2R_RAGGED_LAYER_ORDER, 256},
-                        {1, 256}, {2, 256}, {4, 256}};
-  for (const Cand& cd : cands) {
-    c->tune = keep;
-    c->tune.ragged_schedule = cd.schedule;
-    c->tune.ragged_threads = cd.threads;
-    float ms = 0.f;
-    const int rc = time_calls(
-        c, st, reps,
-        [&] {
-          return p2r_forward_ragged(c, trk_in, hits, trk_out, blk_offsets, nevts, layer_mask, nlayer, bsize,
-                                    semantics, params, nullptr, P2R_RAGGED_AUTO, stream);
-        },
-        &ms);
-    if (rc != P2R_OK) {
-      c->tune = keep;
-      return rc;
-    }
-    p2r_tune_entry e = entry_of(c->tune, bsize, ms, cd.schedule);
-    e.threads = cd.schedule == P2R_RAGGED_LAYER_ORDER ? cd.threads : 256;
+  const Cand cand[] = {{P2R_RAGGED_LAYER_ORDER, 64}, {P2R_RAGGED_LAYER_ORDER, 128}, {P2R_RAGGED_LAYER_ORDER, 256},
+                       {1, 256}, {2, 256}, {4, 256}};
+  constexpr int n = sizeof(cand) / sizeof(cand[0]);
+  float ms[n] = {};
+  const int rc = time_candidates(
+      c, st, reps, n,
+      [&](int i) {
+        c->tune = keep;
+        c->tune.ragged_schedule = cand[i].schedule;
+        c->tune.ragged_threads = cand[i].threads;
+      },
+      [&] {
+        return p2r_forward_ragged(c, trk_in, hits, trk_out, blk_offsets, nevts, layer_mask, nlayer, bsize, semantics,
+                                  params, nullptr, P2R_RAGGED_AUTO, stream);
+      },
+      ms);
+  c->tune = keep;
+  if (rc != P2R_OK) return rc;
+  Collect col;
+  for (int i = 0; i < n; ++i) {
+    p2r_tune_entry e = entry_of(keep, bsize, ms[i], cand[i].schedule);
+    e.threads = cand[i].threads;
     col.add(e);
   }
   col.finish(report);
   const p2r_tune_entry& b = col.rep.entry[col.rep.best];
-  c->tune = keep;
   c->tune.ragged_schedule = b.schedule;
   if (b.schedule == P2R_RAGGED_LAYER_ORDER) c->tune.ragged_threads = b.threads;
   return P2R_OK;
@@ -183,28 +212,25 @@ int p2r_autotune_adjoint(p2r_ctx* c, const void* trk_in, const void* hits, void*
                          p2r_tune_report* report, void* stream) {
   if (!c) return P2R_ERR_ARG;
   if (!s || reps < 1) return c->fail(P2R_ERR_ARG, "null shape or reps < 1");
-  const p2r_tuning keep = c->tune;
   hipStream_t st = static_cast<hipStream_t>(stream);
+  const int strides[4] = {1, 2, 4, 8};
+  int cs = strides[0];
+  float ms[4] = {};
+  const int rc = time_candidates(
+      c, st, reps, 4, [&](int i) { cs = strides[i]; },
+      [&] {
+        return p2r_forward_adjoint(c, trk_in, hits, trk_out, s, params, grad_precision, cs, chi2, grad_par,
+                                   grad_hits, grad_kfact, grad_global, stream);
+      },
+      ms);
+  if (rc != P2R_OK) return rc;
   Collect col;
-  for (int cs : {1, 2, 4, 8}) {
-    float ms = 0.f;
-    const int rc = time_calls(
-        c, st, reps,
-        [&] {
-          return p2r_forward_adjoint(c, trk_in, hits, trk_out, s, params, grad_precision, cs, chi2, grad_par,
-                                     grad_hits, grad_kfact, grad_global, stream);
-        },
-        &ms);
-    if (rc != P2R_OK) {
-      c->tune = keep;
-      return rc;
-    }
-    p2r_tuning t = keep;
-    t.checkpoint_stride = cs;
-    col.add(entry_of(t, s->bsize, ms, 0));
+  for (int i = 0; i < 4; ++i) {
+    p2r_tuning t = c->tune;
+    t.checkpoint_stride = strides[i];
+    col.add(entry_of(t, s->bsize, ms[i], 0));
   }
   col.finish(report);
-  c->tune = keep;
   c->tune.checkpoint_stride = col.rep.entry[col.rep.best].checkpoint_stride;
   return P2R_OK;
 }
@@ -249,7 +275,7 @@ int p2r_autotune_width(p2r_ctx* c, const void* trk_in, const void* hits, const p
     return c->fail(P2R_ERR_NOMEM, "autotune_width copies (%zu bytes)", 2 * tb + hb);
   }
   Collect col;
-  int rc = P2R_OK;
+  int rc = warm_up(c, st, [&] { return p2r_forward(c, trk_in, hits, o, s, semantics, params, nullptr, stream); });
   for (int i = 0; rc == P2R_OK && i < nwidths; ++i) {
     const int w = widths[i];
     rc = p2r_repack(c, trk_in, s->bsize, t, w, s->nevts, s->ntrks, 1, 28, stream);

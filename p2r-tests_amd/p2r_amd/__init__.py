@@ -79,6 +79,17 @@ class Context:
                                _stream(stream))
         self._check(st, "p2r_forward")
 
+    def forward_streamed(self, host_trk, host_hits, host_out, dev_trk, dev_hits, dev_out, shape,
+                         semantics="reference", params=None, precision=None, nstreams=4):
+        """Host-to-host forward, chunks over `nstreams` streams (include/p2r.h p2r_forward_streamed); host_*:
+        pinned CPU tensors, dev_*: device tensors of the same sizes.  Returns when the output is on the host."""
+        sem = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
+        sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
+        p = params if params is not None else default_params(precision or "exact")
+        st = lib().p2r_forward_streamed(self._h, _ptr(host_trk), _ptr(host_hits), _ptr(host_out), _ptr(dev_trk),
+                                        _ptr(dev_hits), _ptr(dev_out), C.byref(sh), sem, C.byref(p), int(nstreams))
+        self._check(st, "p2r_forward_streamed")
+
     def forward_ragged(self, trk, hits, out, blk_offsets, layer_mask, nlayer, bsize, semantics="chained",
                        params=None, chi2=None, stream=None, precision=None, tile_rounds=0):
         """Uneven batch, one persistent launch (include/p2r.h p2r_forward_ragged).  blk_offsets: host int64

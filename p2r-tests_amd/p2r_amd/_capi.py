@@ -80,6 +80,8 @@ def lib():
             "p2r_gen_ragged_layout": ([vp, i32, i32, vp, vp], C.c_int),
             "p2r_gen_ragged": ([vp, vp, vp, vp, i32, vp, i32, i32, i32, i32, C.POINTER(p2r_gen_config)], C.c_int),
             "p2r_gen_lognormal_counts": ([vp, i32, C.c_double, C.c_double, i32, i32, C.c_uint64], C.c_int),
+            "p2r_forward_streamed": ([vp, vp, vp, vp, vp, vp, vp, C.POINTER(p2r_shape), C.c_int,
+                                      C.POINTER(p2r_params), i32], C.c_int),
             "p2r_get_tuning": ([vp, C.POINTER(p2r_tuning)], C.c_int),
             "p2r_set_tuning": ([vp, C.POINTER(p2r_tuning)], C.c_int),
             "p2r_autotune_forward": ([vp, vp, vp, vp, C.POINTER(p2r_shape), C.c_int, C.POINTER(p2r_params), i32,

@@ -143,3 +143,17 @@ def test_autotune_ragged_and_adjoint(ctx):
     ents = rep.as_list()
     assert [e["checkpoint_stride"] for e in ents] == [1, 2, 4, 8]
     assert ctx.get_tuning().checkpoint_stride == ents[rep.best]["checkpoint_stride"]
+
+
+@pytest.mark.parametrize("nstreams", [1, 3, 5])
+def test_streamed_host_to_host_equals_forward(ctx, batch, nstreams):
+    """p2r_forward_streamed (chunks of blocks over streams, pinned host buffers in and out) returns the
+    same words as p2r_forward on device buffers; 96 blocks over 5 streams leaves uneven chunks."""
+    shape, dt, dh = batch
+    for sem in ("reference", "chained"):
+        want, _ = _fwd(ctx, dt, dh, shape, sem)
+        ht, hh = dt.cpu().pin_memory(), dh.cpu().pin_memory()
+        ho = torch.zeros_like(ht).pin_memory()
+        d_in, d_hit, d_out = torch.empty_like(dt), torch.empty_like(dh), torch.empty_like(dt)
+        ctx.forward_streamed(ht, hh, ho, d_in, d_hit, d_out, shape, sem, nstreams=nstreams)
+        assert np.array_equal(ho.numpy().view(np.uint32), want), sem
