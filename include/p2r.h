@@ -70,7 +70,19 @@ typedef struct p2r_params {
   float kfact;
   float b_scale;
   int32_t precision;
+  int32_t options; /* bitwise OR of p2r_option; 0 (the reference's behaviour) by default */
 } p2r_params;
+
+/* Physics the reference leaves commented out after the parameter update
+ * (tbb.cpp:527 note on the charge, :635 squashPhiMPlex), applied to every
+ * layer's updated state when set; forward entry points only (the adjoint
+ * returns P2R_ERR_UNSUPPORTED for options != 0).
+ * P2R_OPT_WRAP_PHI: phi <- phi - 2pi floor((phi + pi) / 2pi), i.e. into
+ *   [-pi, pi) up to FP32 rounding (the multiply by 1/2pi is an FP32 constant).
+ * P2R_OPT_CHARGE_FROM_IPT: an update that makes 1/pt negative flips the charge
+ *   and 1/pt's sign; the next layer propagates with the new charge, and the
+ *   output block carries it. */
+typedef enum p2r_option { P2R_OPT_WRAP_PHI = 1, P2R_OPT_CHARGE_FROM_IPT = 2 } p2r_option;
 
 /* Context: device binding and workspace.  Replaces the reference's
  * p2r_get_compute_device_id / p2r_get_streams set-up

@@ -41,9 +41,9 @@ class Oracle:
         L.orc_KalmanUpdate_block.argtypes = [i, vp, vp, vp, vp]
         L.orc_propagate_lane_f64.argtypes = [vp, vp, i, vp, d, vp, vp, vp]
         L.orc_kalman_lane_f64.argtypes = [vp, vp, vp, vp, vp]
-        L.orc_run_f32.argtypes = [vp, vp, vp, l, i, i, i, f, i, vp, vp]
+        L.orc_run_f32.argtypes = [vp, vp, vp, l, i, i, i, f, i, vp, vp, i]
         L.orc_run_f32.restype = i
-        L.orc_run_f64.argtypes = [vp, vp, vp, vp, vp, l, i, i, i, d, i, vp]
+        L.orc_run_f64.argtypes = [vp, vp, vp, vp, vp, l, i, i, i, d, i, vp, i]
         L.orc_run_f64.restype = i
         L.orc_kfact_tbb.restype = f
         L.orc_gen_c0.argtypes = [vp, vp, i, i, i, i]
@@ -89,26 +89,27 @@ class Oracle:
 
     # -- whole batches in the reference Matriplex layout --
     def run_f32(self, trk, hits, nblk, nlayer, bs, semantics=0, kfact=None, nthreads=0, chi2=False, mask=None,
-                cr_trig=False):
-        """cr_trig: sin/cos correctly rounded (FP64 rounded to float) instead of libm's sinf/cosf."""
+                cr_trig=False, options=0):
+        """cr_trig: sin/cos correctly rounded (FP64 rounded to float) instead of libm's sinf/cosf.
+        options: 1 = wrap phi after each update, 2 = charge from the sign of 1/pt (see run_block)."""
         kfact = self.kfact_tbb() if kfact is None else kfact
         self.lib.orc_set_trig_mode(1 if cr_trig else 0)
         out = np.zeros_like(trk)
         c2 = np.zeros(nblk * bs, np.float32) if chi2 else None
         m = None if mask is None else np.ascontiguousarray(mask, np.uint32)
         rc = self.lib.orc_run_f32(_ptr(trk), _ptr(hits), _ptr(out), nblk, nlayer, bs, semantics, kfact, nthreads,
-                                  _ptr(c2), _ptr(m))
+                                  _ptr(c2), _ptr(m), int(options))
         self.lib.orc_set_trig_mode(0)
         assert rc == 0
         return (out, c2) if chi2 else out
 
-    def run_f64(self, trk, hits, nblk, nlayer, bs, semantics=0, kfact=None, nthreads=0, mask=None):
+    def run_f64(self, trk, hits, nblk, nlayer, bs, semantics=0, kfact=None, nthreads=0, mask=None, options=0):
         kfact = float(self.kfact_tbb()) if kfact is None else kfact
         n = nblk * bs
         par, cov, c2 = np.zeros((n, 6)), np.zeros((n, 21)), np.zeros(n)
         m = None if mask is None else np.ascontiguousarray(mask, np.uint32)
         rc = self.lib.orc_run_f64(_ptr(trk), _ptr(hits), _ptr(par), _ptr(cov), _ptr(c2), nblk, nlayer, bs,
-                                  semantics, kfact, nthreads, _ptr(m))
+                                  semantics, kfact, nthreads, _ptr(m), int(options))
         assert rc == 0
         return par, cov, c2
 

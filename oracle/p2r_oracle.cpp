@@ -50,13 +50,23 @@ inline void store_trk(float* blk, int bs, int it, const Trk<float>& t, int q) {
 // L starts from layer L-1's updated state, propagate-tor-test_hip.cu:839-845).
 // mask (nullable): per-lane bitmask, bit L set = layer L present; absent layers
 // are skipped (chained semantics only).
+// options (bitwise): 1 = wrap phi after each update, phi - 2pi floor((phi + pi) * (1/2pi)) in T
+// (the squash the reference leaves commented out, tbb.cpp:635); 2 = an update that makes 1/pt negative
+// flips 1/pt's sign and the charge (tbb.cpp:527 note), and later layers use the new charge.
+template <class T>
+T wrap_phi(T phi) {
+  const T n = std::floor((phi + T(3.14159265358979323846)) * T(0.15915494309189533577));
+  return phi - n * T(6.28318530717958647693);
+}
+
 template <class T>
 void run_block(const float* tb, const float* hb0, int bs, int nlayer, int semantics, T kf,
-               const uint32_t* mask, Trk<T>* out, int* qout, T* chi2) {
+               const uint32_t* mask, Trk<T>* out, int* qout, T* chi2, int options = 0) {
   for (int it = 0; it < bs; ++it) {
     Trk<T> in;
-    int q;
-    load_trk(tb, bs, it, in, q);
+    int q_in;
+    load_trk(tb, bs, it, in, q_in);
+    int q = q_in;
     Trk<T> cur = in;
     T c2 = T(0.f);
     for (int l = 0; l < nlayer; ++l) {
@@ -64,10 +74,16 @@ void run_block(const float* tb, const float* hb0, int bs, int nlayer, int semant
       Hit<T> h;
       load_hit(hb0 + (long)l * 9 * bs, bs, it, h);
       const Trk<T>& src = (semantics == 0) ? in : cur;
+      if (semantics == 0) q = q_in;
       Trk<T> prop;
       propagate_to_r(src, q, h.pos, kf, prop);
       T c2l;
       kalman_update(prop, h, &c2l);
+      if (options & 1) prop.par[4] = wrap_phi(prop.par[4]);
+      if ((options & 2) && prop.par[3] < T(0.f)) {
+        prop.par[3] = -prop.par[3];
+        q = -q;
+      }
       c2 = c2 + c2l;
       cur = prop;
     }
@@ -169,7 +185,7 @@ void orc_set_trig_mode(int mode) { trig_mode() = mode ? 1 : 0; }
 // receives per-track sum of local chi2 over the processed layers, indexed by
 // block*bs + lane.  mask (nullable) = per-track layer bitmask, same indexing.
 int orc_run_f32(const float* trk, const float* hits, float* out, long nblk, int nlayer, int bs, int semantics,
-                float kfact, int nthreads, float* chi2, const uint32_t* mask) {
+                float kfact, int nthreads, float* chi2, const uint32_t* mask, int options) {
   if (bs <= 0 || nlayer <= 0 || nblk < 0) return 1;
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -179,7 +195,7 @@ int orc_run_f32(const float* trk, const float* hits, float* out, long nblk, int 
     std::vector<Trk<float>> res(bs);
     std::vector<int> q(bs);
     run_block<float>(trk_block(trk, b, bs), hit_block(hits, b * nlayer, bs), bs, nlayer, semantics, kfact,
-                     mask ? mask + b * bs : nullptr, res.data(), q.data(), chi2 ? chi2 + b * bs : nullptr);
+                     mask ? mask + b * bs : nullptr, res.data(), q.data(), chi2 ? chi2 + b * bs : nullptr, options);
     float* ob = trk_block(out, b, bs);
     for (int it = 0; it < bs; ++it) store_trk(ob, bs, it, res[it], q[it]);
   }
@@ -189,7 +205,7 @@ int orc_run_f32(const float* trk, const float* hits, float* out, long nblk, int 
 // FP64 replay: the same algorithm in double precision on the same FP32 inputs.
 // Outputs per track (index block*bs + lane): par_out[6], cov_out[21], chi2.
 int orc_run_f64(const float* trk, const float* hits, double* par_out, double* cov_out, double* chi2, long nblk,
-                int nlayer, int bs, int semantics, double kfact, int nthreads, const uint32_t* mask) {
+                int nlayer, int bs, int semantics, double kfact, int nthreads, const uint32_t* mask, int options) {
   if (bs <= 0 || nlayer <= 0 || nblk < 0) return 1;
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -200,7 +216,7 @@ int orc_run_f64(const float* trk, const float* hits, double* par_out, double* co
     std::vector<int> q(bs);
     std::vector<double> c2(bs);
     run_block<double>(trk_block(trk, b, bs), hit_block(hits, b * nlayer, bs), bs, nlayer, semantics, kfact,
-                      mask ? mask + b * bs : nullptr, res.data(), q.data(), c2.data());
+                      mask ? mask + b * bs : nullptr, res.data(), q.data(), c2.data(), options);
     for (int it = 0; it < bs; ++it) {
       const long t = b * bs + it;
       for (int f = 0; f < 6; ++f) par_out[t * 6 + f] = res[it].par[f];

@@ -11,7 +11,8 @@
 //
 //   p2r_bench [--ntrks N] [--nevts N] [--nlayer N] [--bsize N] [--NITER N]
 //             [--NWARMUP N] [--semantics reference|chained]
-//             [--input reference|baseline] [--include-transfers] [--num-streams N]
+//             [--input reference|baseline] [--transfers none|h2d|d2h|both]
+//             [--include-transfers (= both)] [--num-streams N] [--threads 64|128|256]
 //             [--device D]
 #include <hip/hip_runtime.h>
 
@@ -49,7 +50,8 @@ namespace {
 struct Opts {
   int ntrks = 9600, nevts = 100, nlayer = 20, bsize = 32, niter = 5, nwarmup = 2, device = 0, nstreams = 1;
   int semantics = P2R_REFERENCE;
-  bool reference_input = true, transfers = false;
+  int threads = 0;  // workgroup size of the forward kernel (0 = library default)
+  bool reference_input = true, h2d = false, d2h = false;
 };
 
 Opts parse(int argc, char** argv) {
@@ -72,8 +74,17 @@ Opts parse(int argc, char** argv) {
     else if (a == "--device") o.device = std::atoi(next());
     else if (a == "--semantics") o.semantics = std::string(next()) == "chained" ? P2R_CHAINED : P2R_REFERENCE;
     else if (a == "--input") o.reference_input = std::string(next()) != "baseline";
-    else if (a == "--include-transfers") o.transfers = true;
-    else if (a == "--num-streams") o.nstreams = std::atoi(next());
+    else if (a == "--include-transfers") o.h2d = o.d2h = true;
+    else if (a == "--transfers") {
+      const std::string v = next();
+      o.h2d = v == "both" || v == "h2d";
+      o.d2h = v == "both" || v == "d2h";
+      if (v != "both" && v != "h2d" && v != "d2h" && v != "none") {
+        std::fprintf(stderr, "--transfers takes none, h2d, d2h or both\n");
+        std::exit(2);
+      }
+    } else if (a == "--num-streams") o.nstreams = std::atoi(next());
+    else if (a == "--threads") o.threads = std::atoi(next());
     else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       std::exit(1);
@@ -203,13 +214,27 @@ int main(int argc, char** argv) {
   std::printf("Size of struct struct MPHIT hit[] = %ld\n", (long)(hit_floats * 4));
 
   const p2r_params params = p2r_default_params();
-  // with transfers: the reference's streamed form (hip.cu:941-1013), chunks of blocks over num_streams streams
+  if (o.threads) {
+    p2r_tuning t;
+    CHECK_P2R(ctx, p2r_get_tuning(ctx, &t));
+    t.fwd_threads = o.threads;
+    CHECK_P2R(ctx, p2r_set_tuning(ctx, &t));
+  }
+  // Timed region as the reference's MEASURE_H2D_TRANSFER / MEASURE_D2H_TRANSFER (hip.cu:17-22,958-1006):
+  // both: the streamed form, chunks of blocks over num_streams streams; one of them: that copy on the
+  // stream around the kernel (the other happens outside the timed region); none: the kernel only.
   auto one = [&]() {
-    if (o.transfers)
+    if (o.h2d && o.d2h) {
       CHECK_P2R(ctx, p2r_forward_streamed(ctx, h_trk, h_hit, h_out, d_trk, d_hit, d_out, &shape, o.semantics, &params,
                                           o.nstreams));
-    else
-      CHECK_P2R(ctx, p2r_forward(ctx, d_trk, d_hit, d_out, &shape, o.semantics, &params, nullptr, st));
+      return;
+    }
+    if (o.h2d) {
+      CHECK_HIP(hipMemcpyAsync(d_trk, h_trk, trk_floats * 4, hipMemcpyHostToDevice, st));
+      CHECK_HIP(hipMemcpyAsync(d_hit, h_hit, hit_floats * 4, hipMemcpyHostToDevice, st));
+    }
+    CHECK_P2R(ctx, p2r_forward(ctx, d_trk, d_hit, d_out, &shape, o.semantics, &params, nullptr, st));
+    if (o.d2h) CHECK_HIP(hipMemcpyAsync(h_out, d_out, trk_floats * 4, hipMemcpyDeviceToHost, st));
   };
   for (int i = 0; i < o.nwarmup; ++i) one();
   CHECK_HIP(hipStreamSynchronize(st));

@@ -51,15 +51,16 @@ template <int SEM, bool CHI2, bool EXACT>
 __global__ __launch_bounds__(kThreads) void fwd_uniform_kernel(const float* trk, const float* __restrict__ hits,
                                                                float* out, float* __restrict__ chi2, uint32_t ntrk,
                                                                uint32_t bs, int nlayer, float kfact, int l0, int l1,
-                                                               bool acc) {
+                                                               bool acc, int opts) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ntrk) return;
   const uint32_t blk = g / bs, lane = g - blk * bs;
   const float* tb = trk + (size_t)blk * 28 * bs + lane;
   Trk in;
   load_trk(tb, bs, in);
-  const int q = reinterpret_cast<const int*>(tb)[(size_t)27 * bs];
-  const float k = (float)q * kfact;  // tbb.cpp:667
+  const int q_in = reinterpret_cast<const int*>(tb)[(size_t)27 * bs];
+  int q = q_in;
+  float k = (float)q * kfact;  // tbb.cpp:667
 
   const float* hb = hits + (size_t)blk * nlayer * 9 * bs + lane;
   const size_t hstride = (size_t)9 * bs;
@@ -75,12 +76,15 @@ __global__ __launch_bounds__(kThreads) void fwd_uniform_kernel(const float* trk,
     if (SEM == P2R_REFERENCE) {
       src = in;      // tbb.cpp:894: every layer starts from the input track
       opaque(src);   // recompute per layer; nothing hoisted across layers
+      q = q_in;
+      k = (float)q * kfact;
     } else {
       src = cur;     // propagate-tor-test_hip.cu:842-845
     }
     Trk prop;
     Lane<EXACT>::propagate(src, k, h.pos[0], h.pos[1], prop);
     float c = Lane<EXACT>::update(prop, h);
+    if (opts) apply_options<EXACT>(opts, prop, q, k, kfact);
     if (SEM == P2R_REFERENCE) {
       // Only the last layer's result is returned in reference semantics; the
       // barrier makes every layer's result observable so the compiler cannot
@@ -112,13 +116,14 @@ constexpr int kRagWaves = kRagThreads / 64;
 template <int SEM, bool CHI2, bool EXACT>
 __device__ __forceinline__ void ragged_slot(const float* __restrict__ trk, const float* __restrict__ hits,
                                             float* __restrict__ out, float* __restrict__ chi2, uint32_t slot,
-                                            uint32_t m, uint32_t bs, int nlayer, float kfact) {
+                                            uint32_t m, uint32_t bs, int nlayer, float kfact, int opts) {
   const uint32_t blk = slot / bs, lane = slot - blk * bs;
   const float* tb = trk + (size_t)blk * 28 * bs + lane;
   Trk in;
   load_trk(tb, bs, in);
-  const int q = reinterpret_cast<const int*>(tb)[(size_t)27 * bs];
-  const float k = (float)q * kfact;
+  const int q_in = reinterpret_cast<const int*>(tb)[(size_t)27 * bs];
+  int q = q_in;
+  float k = (float)q * kfact;
   const float* hb = hits + (size_t)blk * nlayer * 9 * bs + lane;
   const size_t hstride = (size_t)9 * bs;
   Trk cur = in;
@@ -133,12 +138,15 @@ __device__ __forceinline__ void ragged_slot(const float* __restrict__ trk, const
     if (SEM == P2R_REFERENCE) {
       src = in;
       opaque(src);
+      q = q_in;
+      k = (float)q * kfact;
     } else {
       src = cur;
     }
     Trk prop;
     Lane<EXACT>::propagate(src, k, h.pos[0], h.pos[1], prop);
     float c = Lane<EXACT>::update(prop, h);
+    if (opts) apply_options<EXACT>(opts, prop, q, k, kfact);
     if (SEM == P2R_REFERENCE) {
       opaque(prop);
       opaque(c);
@@ -158,7 +166,7 @@ __global__ __launch_bounds__(kRagThreads) void fwd_ragged_kernel(const float* __
                                                                  float* __restrict__ out, float* __restrict__ chi2,
                                                                  const uint32_t* __restrict__ mask, uint32_t nslot,
                                                                  uint32_t bs, int nlayer, float kfact,
-                                                                 uint32_t ntiles) {
+                                                                 uint32_t ntiles, int opts) {
   constexpr int kTile = kRagThreads * ROUNDS;
   __shared__ uint32_t hist[kRagWaves * ROUNDS][33];  // per (wave, round): tracks with each hit count
   __shared__ uint32_t start[33];                       // first rank of each hit count in the tile
@@ -215,7 +223,7 @@ __global__ __launch_bounds__(kRagThreads) void fwd_ragged_kernel(const float* __
       const uint32_t rank = r * kRagThreads + tid;
       if (rank < nvalid) {
         const uint32_t local = order[rank];
-        ragged_slot<SEM, CHI2, EXACT>(trk, hits, out, chi2, base + local, smask[local], bs, nlayer, kfact);
+        ragged_slot<SEM, CHI2, EXACT>(trk, hits, out, chi2, base + local, smask[local], bs, nlayer, kfact, opts);
       }
     }
     __syncthreads();  // order/smask/hist are rewritten by the next tile
@@ -234,7 +242,7 @@ __global__ __launch_bounds__(kThreads) void fwd_ragged_layer_kernel(const float*
                                                                     const float* __restrict__ hits,
                                                                     float* __restrict__ out, float* __restrict__ chi2,
                                                                     const uint32_t* __restrict__ mask, uint32_t nslot,
-                                                                    uint32_t bs, int nlayer, float kfact) {
+                                                                    uint32_t bs, int nlayer, float kfact, int opts) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;  // workgroups of 64..256 lanes
   const bool active = g < nslot;
   // bits at or above nlayer name no hit block: ignored
@@ -249,8 +257,9 @@ __global__ __launch_bounds__(kThreads) void fwd_ragged_layer_kernel(const float*
   const float* tb = trk + (size_t)blk * 28 * bs + lane;
   Trk in;
   load_trk(tb, bs, in);
-  const int q = reinterpret_cast<const int*>(tb)[(size_t)27 * bs];
-  const float k = (float)q * kfact;
+  const int q_in = reinterpret_cast<const int*>(tb)[(size_t)27 * bs];
+  int q = q_in;
+  float k = (float)q * kfact;
   const float* hb = hits + (size_t)blk * nlayer * 9 * bs + lane;
   const size_t hstride = (size_t)9 * bs;
   Trk cur = in;
@@ -268,12 +277,15 @@ __global__ __launch_bounds__(kThreads) void fwd_ragged_layer_kernel(const float*
       if (SEM == P2R_REFERENCE) {
         src = in;
         opaque(src);
+        q = q_in;
+        k = (float)q * kfact;
       } else {
         src = cur;
       }
       Trk prop;
       Lane<EXACT>::propagate(src, k, h.pos[0], h.pos[1], prop);
       float c = Lane<EXACT>::update(prop, h);
+      if (opts) apply_options<EXACT>(opts, prop, q, k, kfact);
       if (SEM == P2R_REFERENCE) {
         opaque(prop);
         opaque(c);
@@ -542,16 +554,17 @@ struct UniformLaunch {
   uint32_t n, bs;
   int nlayer;
   float kf;
+  int opts;
 };
 template <int SEM, bool CHI2>
 void launch_uniform_p(bool exact, dim3 grid, dim3 block, hipStream_t st, const UniformLaunch& a, const float* in,
                       int l0, int l1, bool acc) {
   if (exact)
     fwd_uniform_kernel<SEM, CHI2, true><<<grid, block, 0, st>>>(in, a.H, a.O, a.chi2, a.n, a.bs, a.nlayer, a.kf, l0,
-                                                                l1, acc);
+                                                                l1, acc, a.opts);
   else
     fwd_uniform_kernel<SEM, CHI2, false><<<grid, block, 0, st>>>(in, a.H, a.O, a.chi2, a.n, a.bs, a.nlayer, a.kf, l0,
-                                                                 l1, acc);
+                                                                 l1, acc, a.opts);
 }
 // The whole forward: one launch over all layers, or launches of `per_launch`
 // layers (the state passes through trk_out between launches; in reference
@@ -576,7 +589,7 @@ void launch_uniform(int sem, bool exact, int threads, int per_launch, hipStream_
 
 template <int SEM, bool CHI2, bool EXACT, int ROUNDS>
 int launch_ragged_t(p2r_ctx* c, hipStream_t st, const float* T, const float* H, float* O, float* chi2,
-                    const uint32_t* M, uint32_t nslot, uint32_t bs, int nlayer, float kf) {
+                    const uint32_t* M, uint32_t nslot, uint32_t bs, int nlayer, float kf, int opts) {
   constexpr int kTile = kRagThreads * ROUNDS;
   const uint32_t ntiles = (nslot + kTile - 1) / kTile;
   int per_cu = 0;
@@ -585,33 +598,33 @@ int launch_ragged_t(p2r_ctx* c, hipStream_t st, const float* T, const float* H, 
     per_cu = 1;
   const uint32_t grid = std::min<uint32_t>(ntiles, (uint32_t)(c->num_cus * per_cu));
   fwd_ragged_kernel<SEM, CHI2, EXACT, ROUNDS><<<grid, kRagThreads, 0, st>>>(T, H, O, chi2, M, nslot, bs, nlayer, kf,
-                                                                           ntiles);
+                                                                           ntiles, opts);
   return P2R_OK;
 }
 template <int SEM, bool CHI2, bool EXACT>
 int launch_ragged_r(int rounds, p2r_ctx* c, hipStream_t st, const float* T, const float* H, float* O, float* chi2,
-                    const uint32_t* M, uint32_t nslot, uint32_t bs, int nlayer, float kf) {
+                    const uint32_t* M, uint32_t nslot, uint32_t bs, int nlayer, float kf, int opts) {
   if (rounds == P2R_RAGGED_LAYER_ORDER) {
     const int threads = c->tune.ragged_threads;
     fwd_ragged_layer_kernel<SEM, CHI2, EXACT><<<grid_for_n(nslot, threads), threads, 0, st>>>(T, H, O, chi2, M, nslot,
-                                                                                            bs, nlayer, kf);
+                                                                                            bs, nlayer, kf, opts);
     return P2R_OK;
   }
   switch (rounds) {
-    case 1: return launch_ragged_t<SEM, CHI2, EXACT, 1>(c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
-    case 2: return launch_ragged_t<SEM, CHI2, EXACT, 2>(c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
-    default: return launch_ragged_t<SEM, CHI2, EXACT, 4>(c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
+    case 1: return launch_ragged_t<SEM, CHI2, EXACT, 1>(c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf, opts);
+    case 2: return launch_ragged_t<SEM, CHI2, EXACT, 2>(c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf, opts);
+    default: return launch_ragged_t<SEM, CHI2, EXACT, 4>(c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf, opts);
   }
 }
 template <int SEM>
 int launch_ragged_s(bool chi2_on, bool exact, int rounds, p2r_ctx* c, hipStream_t st, const float* T, const float* H,
-                    float* O, float* chi2, const uint32_t* M, uint32_t nslot, uint32_t bs, int nlayer, float kf) {
+                    float* O, float* chi2, const uint32_t* M, uint32_t nslot, uint32_t bs, int nlayer, float kf, int opts) {
   if (chi2_on) {
-    if (exact) return launch_ragged_r<SEM, true, true>(rounds, c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
-    return launch_ragged_r<SEM, true, false>(rounds, c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
+    if (exact) return launch_ragged_r<SEM, true, true>(rounds, c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf, opts);
+    return launch_ragged_r<SEM, true, false>(rounds, c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf, opts);
   }
-  if (exact) return launch_ragged_r<SEM, false, true>(rounds, c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
-  return launch_ragged_r<SEM, false, false>(rounds, c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf);
+  if (exact) return launch_ragged_r<SEM, false, true>(rounds, c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf, opts);
+  return launch_ragged_r<SEM, false, false>(rounds, c, st, T, H, O, chi2, M, nslot, bs, nlayer, kf, opts);
 }
 
 template <class RS, class RL, bool EXACT, int CS>
@@ -645,6 +658,7 @@ p2r_params p2r_default_params(void) {
   p.kfact = (float)(100 / (-0.299792458 * 3.8112));  // tbb.cpp:654
   p.b_scale = 1.f;
   p.precision = P2R_EXACT;
+  p.options = 0;
   return p;
 }
 
@@ -709,9 +723,10 @@ int p2r_forward(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out,
   if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
   if (ntrk == 0) return P2R_OK;
   if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
+  if (p.options & ~(P2R_OPT_WRAP_PHI | P2R_OPT_CHARGE_FROM_IPT)) return c->fail(P2R_ERR_ARG, "bad options %d", p.options);
   DeviceGuard g(c->device);
   const UniformLaunch a{static_cast<const float*>(trk_in), static_cast<const float*>(hits), static_cast<float*>(trk_out),
-                        chi2, (uint32_t)ntrk, (uint32_t)s->bsize, s->nlayer, p.kfact / p.b_scale};
+                        chi2, (uint32_t)ntrk, (uint32_t)s->bsize, s->nlayer, p.kfact / p.b_scale, p.options};
   launch_uniform(semantics, p.precision == P2R_EXACT, c->tune.fwd_threads, c->tune.fwd_layers_per_launch,
                  static_cast<hipStream_t>(stream), a);
   return c->check_launch("p2r_forward");
@@ -734,6 +749,7 @@ int p2r_forward_streamed(p2r_ctx* c, const void* host_trk_in, const void* host_h
   const p2r_params p = params ? *params : p2r_default_params();
   if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
   if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
+  if (p.options & ~(P2R_OPT_WRAP_PHI | P2R_OPT_CHARGE_FROM_IPT)) return c->fail(P2R_ERR_ARG, "bad options %d", p.options);
   if (nblk == 0) return P2R_OK;
   DeviceGuard g(c->device);
   const size_t bs = (size_t)s->bsize, tw = 28 * bs * 4, hw = (size_t)s->nlayer * 9 * bs * 4;  // bytes per block
@@ -753,7 +769,7 @@ int p2r_forward_streamed(p2r_ctx* c, const void* host_trk_in, const void* host_h
       return c->fail(P2R_ERR_HIP, "host-to-device copy of chunk %d failed", k);
     const UniformLaunch a{reinterpret_cast<const float*>(dt), reinterpret_cast<const float*>(dh),
                           reinterpret_cast<float*>(dout), nullptr, (uint32_t)(nb * bs), (uint32_t)bs, s->nlayer,
-                          p.kfact / p.b_scale};
+                          p.kfact / p.b_scale, p.options};
     launch_uniform(semantics, p.precision == P2R_EXACT, c->tune.fwd_threads, c->tune.fwd_layers_per_launch, st, a);
     const int rc = c->check_launch("p2r_forward_streamed");
     if (rc != P2R_OK) return rc;
@@ -787,6 +803,7 @@ int p2r_forward_ragged(p2r_ctx* c, const void* trk_in, const void* hits, void* t
   const p2r_params p = params ? *params : p2r_default_params();
   if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
   if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
+  if (p.options & ~(P2R_OPT_WRAP_PHI | P2R_OPT_CHARGE_FROM_IPT)) return c->fail(P2R_ERR_ARG, "bad options %d", p.options);
   if (nslot == 0) return P2R_OK;
   const int rounds = tile_rounds != P2R_RAGGED_AUTO ? tile_rounds : c->tune.ragged_schedule;
   DeviceGuard g(c->device);
@@ -798,10 +815,10 @@ int p2r_forward_ragged(p2r_ctx* c, const void* trk_in, const void* hits, void* t
   const bool exact = p.precision == P2R_EXACT;
   if (semantics == P2R_REFERENCE)
     launch_ragged_s<P2R_REFERENCE>(chi2 != nullptr, exact, rounds, c, st, T, H, O, chi2, layer_mask, (uint32_t)nslot,
-                                   (uint32_t)bsize, nlayer, kf);
+                                   (uint32_t)bsize, nlayer, kf, p.options);
   else
     launch_ragged_s<P2R_CHAINED>(chi2 != nullptr, exact, rounds, c, st, T, H, O, chi2, layer_mask, (uint32_t)nslot,
-                                 (uint32_t)bsize, nlayer, kf);
+                                 (uint32_t)bsize, nlayer, kf, p.options);
   return c->check_launch("p2r_forward_ragged");
 }
 
@@ -825,6 +842,8 @@ int p2r_forward_adjoint(p2r_ctx* c, const void* trk_in, const void* hits, void* 
   const p2r_params p = params ? *params : p2r_default_params();
   if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
   if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
+  if (p.options != 0)
+    return c->fail(P2R_ERR_UNSUPPORTED, "p2r_forward_adjoint differentiates the reference's update only (options 0)");
   if (ntrk == 0) return P2R_OK;
   DeviceGuard g(c->device);
   const int nlayer = s->nlayer, nglob = 4 * nlayer + 1;

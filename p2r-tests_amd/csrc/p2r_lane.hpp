@@ -17,6 +17,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "../../include/p2r.h"
+
 namespace p2r {
 
 constexpr int kNiter = 5;  // fixed-point steps of propagateToR, tbb.cpp:655
@@ -118,12 +120,26 @@ template <> struct Lane<true> {
     exact::propagate_to_r(in, k, hx, hy, out);
   }
   static __device__ __forceinline__ float update(Trk& t, const Hit& h) { return exact::kalman_update(t, h); }
+  static __device__ __forceinline__ float wrap_phi(float phi) { return exact::m_wrap_phi(phi); }
 };
 template <> struct Lane<false> {
   static __device__ __forceinline__ void propagate(const Trk& in, float k, float hx, float hy, Trk& out) {
     fast::propagate_to_r(in, k, hx, hy, out);
   }
   static __device__ __forceinline__ float update(Trk& t, const Hit& h) { return fast::kalman_update(t, h); }
+  static __device__ __forceinline__ float wrap_phi(float phi) { return fast::m_wrap_phi(phi); }
 };
+
+// The options of p2r_params applied to a layer's updated state (tbb.cpp:527, :635): phi wrapped; a
+// negative 1/pt flips its sign and the charge, and k = q kfact follows the charge.
+template <bool EXACT>
+__device__ __forceinline__ void apply_options(int opts, Trk& t, int& q, float& k, float kfact) {
+  if (opts & P2R_OPT_WRAP_PHI) t.par[4] = Lane<EXACT>::wrap_phi(t.par[4]);
+  if ((opts & P2R_OPT_CHARGE_FROM_IPT) && t.par[3] < 0.f) {
+    t.par[3] = -t.par[3];
+    q = -q;
+    k = (float)q * kfact;
+  }
+}
 
 }  // namespace p2r

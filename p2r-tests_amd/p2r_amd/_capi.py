@@ -13,6 +13,7 @@ P2R_REFERENCE, P2R_CHAINED = 0, 1
 P2R_EXACT, P2R_FAST = 0, 1
 P2R_GRAD_F32, P2R_GRAD_MIXED, P2R_GRAD_F64 = 0, 1, 2
 P2R_RAGGED_AUTO, P2R_RAGGED_LAYER_ORDER = 0, -1
+P2R_OPT_WRAP_PHI, P2R_OPT_CHARGE_FROM_IPT = 1, 2
 
 
 class p2r_shape(C.Structure):
@@ -20,7 +21,7 @@ class p2r_shape(C.Structure):
 
 
 class p2r_params(C.Structure):
-    _fields_ = [("kfact", C.c_float), ("b_scale", C.c_float), ("precision", C.c_int32)]
+    _fields_ = [("kfact", C.c_float), ("b_scale", C.c_float), ("precision", C.c_int32), ("options", C.c_int32)]
 
 
 class p2r_gen_config(C.Structure):

@@ -324,3 +324,54 @@ def test_ragged_graph_replays_match_eager(ctx):
         torch.cuda.synchronize()
         assert bit_mismatches(do.cpu().numpy(), eager) == 0, f"replay {i}"
         _ = (dt * 2).sum().item()  # unrelated eager work between replays
+
+
+# ---- options: phi wrap, charge from the sign of 1/pt -------------------------
+
+@pytest.mark.parametrize("options", [1, 2, 3])
+@pytest.mark.parametrize("semantics", ["reference", "chained"])
+def test_exact_options_match_oracle(orc, ctx, semantics, options):
+    """Inputs with phi moved by multiples of 2 pi and tiny 1/pt on some tracks: the options act, and the exact
+    path stays bit-identical to the oracle run with the same options (charge word included)."""
+    import p2r_amd
+    from test_oracle_pins import _options_inputs
+    shape, trk, hits = _options_inputs(seed=options)
+    sem = 0 if semantics == "reference" else 1
+    p = p2r_amd.default_params("exact", options=options)
+    dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
+    do = torch.empty_like(dt)
+    dc = torch.empty(shape.ntrk_total, device="cuda")
+    ctx.forward(dt, dh, do, shape, semantics, params=p, chi2=dc)
+    torch.cuda.synchronize()
+    cpu, c2c = orc.run_f32(trk, hits, shape.nblk, 20, 32, semantics=sem, chi2=True, cr_trig=True, options=options)
+    gpu = do.cpu().numpy()
+    assert bit_mismatches(gpu, cpu) == 0 and bit_mismatches(dc.cpu().numpy(), c2c) == 0
+    base = orc.run_f32(trk, hits, shape.nblk, 20, 32, semantics=sem, cr_trig=True)
+    assert bit_mismatches(base, cpu) > 0  # the options changed something on these inputs
+
+
+@pytest.mark.parametrize("rounds", [-1, 1])
+def test_exact_options_ragged_match_oracle(orc, ctx, rounds):
+    import p2r_amd
+    counts, trk, hits, mask, offs = _ragged_inputs(nevts=10)
+    T = trk.reshape(-1, 28, 32)
+    T[:, 4, :] += np.float32(2 * np.pi)
+    p = p2r_amd.default_params("exact", options=3)
+    dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
+    dm = torch.from_numpy(mask.view(np.int32)).cuda()
+    do = torch.empty_like(dt)
+    ctx.forward_ragged(dt, dh, do, offs, dm, 20, 32, "chained", params=p, tile_rounds=rounds)
+    torch.cuda.synchronize()
+    cpu = orc.run_f32(trk, hits, int(offs[-1]), 20, 32, semantics=1, mask=mask, cr_trig=True, options=3)
+    assert bit_mismatches(do.cpu().numpy(), cpu) == 0
+
+
+def test_options_rejected_where_unsupported(ctx):
+    import p2r_amd
+    shape = Shape(1, 64, 20, 32)
+    trk, hits = p2r_amd.gen_uniform(shape)
+    dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
+    with pytest.raises(p2r_amd.P2RError, match="not supported"):
+        ctx.forward_adjoint(dt, dh, torch.empty_like(dt), shape, params=p2r_amd.default_params("exact", options=1))
+    with pytest.raises(p2r_amd.P2RError):
+        ctx.forward(dt, dh, torch.empty_like(dt), shape, params=p2r_amd.default_params("exact", options=8))

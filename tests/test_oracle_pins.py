@@ -199,3 +199,58 @@ def test_opcount_frozen(orc):
     assert c == {"add": 494, "mul": 757, "div": 12, "sqrt": 8, "trig": 4, "trivial": 63, "chi2": 9}
     total = c["add"] + c["mul"] + c["div"] + c["sqrt"] + c["trig"]
     assert total == 1275 and total - c["trivial"] == 1212
+
+
+def _options_inputs(seed=3):
+    """A uniform batch with phi moved by multiples of 2 pi on some tracks (the wrap then acts) and 1/pt
+    shrunk to ~0 on others (an update can then make it negative)."""
+    import p2r_amd
+    from p2r_amd import Shape
+    shape = Shape(1, 256, 20, 32)
+    trk, hits = p2r_amd.gen_uniform(shape)
+    T = trk.reshape(-1, 28, 32)
+    rng = np.random.default_rng(seed)
+    T[:, 4, :] += (2 * np.pi * rng.integers(-2, 3, T[:, 4, :].shape)).astype(np.float32)
+    small = rng.random(T[:, 3, :].shape) < 0.3
+    T[:, 3, :] = np.where(small, T[:, 3, :] * 1e-3, T[:, 3, :]).astype(np.float32)
+    return shape, trk, hits
+
+
+def test_option_wrap_phi_on_one_layer(orc):
+    """One layer with options=1 equals options=0 except phi, which is phi0 - 2 pi n with the n of the formula,
+    lands in [-pi, pi] and leaves cos/sin unchanged to FP32 rounding."""
+    shape, trk, hits = _options_inputs()
+    h1 = np.ascontiguousarray(hits.reshape(-1, 20, 9, 32)[:, :1]).ravel()
+    o0 = orc.run_f32(trk, h1, shape.nblk, 1, 32, semantics=1).reshape(-1, 28, 32)
+    o1 = orc.run_f32(trk, h1, shape.nblk, 1, 32, semantics=1, options=1).reshape(-1, 28, 32)
+    keep = [f for f in range(28) if f != 4]
+    assert np.array_equal(o0[:, keep].view(np.uint32), o1[:, keep].view(np.uint32))
+    phi0, phi1 = o0[:, 4].astype(np.float64), o1[:, 4].astype(np.float64)
+    assert np.all(phi1 >= -np.pi - 1e-6) and np.all(phi1 <= np.pi + 1e-6)
+    assert np.any(np.abs(phi0) > np.pi + 0.1)  # the inputs exercise the wrap
+    np.testing.assert_allclose(np.cos(phi1), np.cos(phi0), atol=1e-5)
+    np.testing.assert_allclose(np.sin(phi1), np.sin(phi0), atol=1e-5)
+    n = np.round((phi0 - phi1) / (2 * np.pi))
+    np.testing.assert_allclose(phi0 - 2 * np.pi * n, phi1, atol=1e-5)
+
+
+def test_option_charge_from_ipt(orc):
+    """With options=2 a negative updated 1/pt comes out positive with the charge flipped (one layer:
+    everything else equal); over 20 chained layers the flipped tracks then propagate with the new charge,
+    so they differ from options=0, and unflipped tracks stay identical."""
+    shape, trk, hits = _options_inputs()
+    h1 = np.ascontiguousarray(hits.reshape(-1, 20, 9, 32)[:, :1]).ravel()
+    o0 = orc.run_f32(trk, h1, shape.nblk, 1, 32, semantics=1).reshape(-1, 28, 32)
+    o2 = orc.run_f32(trk, h1, shape.nblk, 1, 32, semantics=1, options=2).reshape(-1, 28, 32)
+    neg = o0[:, 3] < 0
+    assert neg.any() and (~neg).any()
+    q0, q2 = o0[:, 27].view(np.int32), o2[:, 27].view(np.int32)
+    assert np.array_equal(q2[neg], -q0[neg]) and np.array_equal(q2[~neg], q0[~neg])
+    assert np.array_equal(o2[:, 3][neg], -o0[:, 3][neg]) and np.array_equal(o2[:, 3][~neg], o0[:, 3][~neg])
+    keep = [f for f in range(27) if f != 3]
+    assert np.array_equal(o0[:, keep].view(np.uint32), o2[:, keep].view(np.uint32))
+    # reference semantics: every layer restarts from the input charge; the last layer's flip is what comes out
+    r2 = orc.run_f32(trk, hits, shape.nblk, 20, 32, semantics=0, options=2).reshape(-1, 28, 32)
+    r0 = orc.run_f32(trk, hits, shape.nblk, 20, 32, semantics=0).reshape(-1, 28, 32)
+    negr = r0[:, 3] < 0
+    assert np.array_equal(r2[:, 27].view(np.int32)[negr], -r0[:, 27].view(np.int32)[negr])
