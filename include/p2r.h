@@ -209,13 +209,16 @@ typedef struct p2r_tuning {
   int32_t ragged_schedule;       /* what P2R_RAGGED_AUTO runs: P2R_RAGGED_LAYER_ORDER (default), 1, 2, 4 */
   int32_t ragged_threads;        /* workgroup size of the layer-ordered schedule: 64, 128 or 256 (256) */
   int32_t checkpoint_stride;     /* what checkpoint_stride = 0 means in p2r_forward_adjoint: 1, 2, 4, 8 (4) */
+  int32_t fwd_waves;             /* p2r_forward kernels compiled for at least this many waves per SIMD:
+                                    0 = the compiler's register allocation (default), 3 or 4 (fewer
+                                    registers, possibly spills) */
 } p2r_tuning;
 int p2r_get_tuning(const p2r_ctx* ctx, p2r_tuning* out);
 int p2r_set_tuning(p2r_ctx* ctx, const p2r_tuning* in); /* P2R_ERR_ARG on a value outside the lists above */
 
 /* One timed candidate: the launch choices and the mean time of one call (ms). */
 typedef struct p2r_tune_entry {
-  int32_t bsize, threads, layers_per_launch, schedule, checkpoint_stride;
+  int32_t bsize, threads, layers_per_launch, schedule, checkpoint_stride, waves;
   float ms;
 } p2r_tune_entry;
 typedef struct p2r_tune_report {
@@ -230,7 +233,8 @@ typedef struct p2r_tune_report {
  * synchronize the stream and create events: not for use under graph capture.
  * Outputs are overwritten with valid results of the last candidate run.
  *   forward: fwd_threads {64, 128, 256} x fwd_layers_per_launch {all,
- *            ceil(nlayer/2), ceil(nlayer/4)};
+ *            ceil(nlayer/2), ceil(nlayer/4)}, then fwd_waves {0, 3, 4} at the
+ *            best of those;
  *   ragged:  ragged_schedule {layer order x ragged_threads {64, 128, 256},
  *            sorted tiles of 1, 2, 4 rounds};
  *   adjoint: checkpoint_stride {1, 2, 4, 8} (grad_precision as given). */

@@ -47,8 +47,10 @@ __device__ __forceinline__ void load_hit(const float* hb, uint32_t bs, Hit& h) {
 // Layers [l0, l1) of nlayer (all of them for one launch; a layer range when the
 // tuning splits the loop over several launches: then chi2 accumulates if acc).
 // Workgroups of 64..256 lanes (blockDim.x).
-template <int SEM, bool CHI2, bool EXACT>
-__global__ __launch_bounds__(kThreads) void fwd_uniform_kernel(const float* trk, const float* __restrict__ hits,
+// MINW: minimum waves per SIMD the register allocation must allow (1 = the compiler's choice; 3 or 4
+// trade spills for occupancy, a tuning choice).
+template <int SEM, bool CHI2, bool EXACT, int MINW>
+__global__ __launch_bounds__(kThreads, MINW) void fwd_uniform_kernel(const float* trk, const float* __restrict__ hits,
                                                                float* out, float* __restrict__ chi2, uint32_t ntrk,
                                                                uint32_t bs, int nlayer, float kfact, int l0, int l1,
                                                                bool acc, int opts) {
@@ -556,20 +558,30 @@ struct UniformLaunch {
   float kf;
   int opts;
 };
-template <int SEM, bool CHI2>
-void launch_uniform_p(bool exact, dim3 grid, dim3 block, hipStream_t st, const UniformLaunch& a, const float* in,
+template <int SEM, bool CHI2, bool EXACT, int MINW>
+void launch_uniform_w(dim3 grid, dim3 block, hipStream_t st, const UniformLaunch& a, const float* in, int l0, int l1,
+                      bool acc) {
+  fwd_uniform_kernel<SEM, CHI2, EXACT, MINW><<<grid, block, 0, st>>>(in, a.H, a.O, a.chi2, a.n, a.bs, a.nlayer, a.kf,
+                                                                     l0, l1, acc, a.opts);
+}
+template <int SEM, bool CHI2, bool EXACT>
+void launch_uniform_e(int waves, dim3 grid, dim3 block, hipStream_t st, const UniformLaunch& a, const float* in,
                       int l0, int l1, bool acc) {
-  if (exact)
-    fwd_uniform_kernel<SEM, CHI2, true><<<grid, block, 0, st>>>(in, a.H, a.O, a.chi2, a.n, a.bs, a.nlayer, a.kf, l0,
-                                                                l1, acc, a.opts);
-  else
-    fwd_uniform_kernel<SEM, CHI2, false><<<grid, block, 0, st>>>(in, a.H, a.O, a.chi2, a.n, a.bs, a.nlayer, a.kf, l0,
-                                                                 l1, acc, a.opts);
+  if (waves == 3) launch_uniform_w<SEM, CHI2, EXACT, 3>(grid, block, st, a, in, l0, l1, acc);
+  else if (waves == 4) launch_uniform_w<SEM, CHI2, EXACT, 4>(grid, block, st, a, in, l0, l1, acc);
+  else launch_uniform_w<SEM, CHI2, EXACT, 1>(grid, block, st, a, in, l0, l1, acc);
+}
+template <int SEM, bool CHI2>
+void launch_uniform_p(bool exact, int waves, dim3 grid, dim3 block, hipStream_t st, const UniformLaunch& a,
+                      const float* in, int l0, int l1, bool acc) {
+  if (exact) launch_uniform_e<SEM, CHI2, true>(waves, grid, block, st, a, in, l0, l1, acc);
+  else launch_uniform_e<SEM, CHI2, false>(waves, grid, block, st, a, in, l0, l1, acc);
 }
 // The whole forward: one launch over all layers, or launches of `per_launch`
 // layers (the state passes through trk_out between launches; in reference
 // semantics every launch restarts from trk_in, as every layer does).
-void launch_uniform(int sem, bool exact, int threads, int per_launch, hipStream_t st, const UniformLaunch& a) {
+void launch_uniform(int sem, bool exact, int threads, int per_launch, int waves, hipStream_t st,
+                    const UniformLaunch& a) {
   const dim3 grid(grid_for_n(a.n, threads)), block(threads);
   const int step = (per_launch > 0 && per_launch < a.nlayer) ? per_launch : a.nlayer;
   for (int l0 = 0; l0 < a.nlayer; l0 += step) {
@@ -577,11 +589,11 @@ void launch_uniform(int sem, bool exact, int threads, int per_launch, hipStream_
     const float* in = (sem == P2R_CHAINED && l0 > 0) ? a.O : a.T;
     const bool acc = l0 > 0;
     if (sem == P2R_REFERENCE) {
-      if (a.chi2) launch_uniform_p<P2R_REFERENCE, true>(exact, grid, block, st, a, in, l0, l1, acc);
-      else launch_uniform_p<P2R_REFERENCE, false>(exact, grid, block, st, a, in, l0, l1, acc);
+      if (a.chi2) launch_uniform_p<P2R_REFERENCE, true>(exact, waves, grid, block, st, a, in, l0, l1, acc);
+      else launch_uniform_p<P2R_REFERENCE, false>(exact, waves, grid, block, st, a, in, l0, l1, acc);
     } else {
-      if (a.chi2) launch_uniform_p<P2R_CHAINED, true>(exact, grid, block, st, a, in, l0, l1, acc);
-      else launch_uniform_p<P2R_CHAINED, false>(exact, grid, block, st, a, in, l0, l1, acc);
+      if (a.chi2) launch_uniform_p<P2R_CHAINED, true>(exact, waves, grid, block, st, a, in, l0, l1, acc);
+      else launch_uniform_p<P2R_CHAINED, false>(exact, waves, grid, block, st, a, in, l0, l1, acc);
     }
   }
 }
@@ -728,7 +740,7 @@ int p2r_forward(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out,
   const UniformLaunch a{static_cast<const float*>(trk_in), static_cast<const float*>(hits), static_cast<float*>(trk_out),
                         chi2, (uint32_t)ntrk, (uint32_t)s->bsize, s->nlayer, p.kfact / p.b_scale, p.options};
   launch_uniform(semantics, p.precision == P2R_EXACT, c->tune.fwd_threads, c->tune.fwd_layers_per_launch,
-                 static_cast<hipStream_t>(stream), a);
+                 c->tune.fwd_waves, static_cast<hipStream_t>(stream), a);
   return c->check_launch("p2r_forward");
 }
 
@@ -770,7 +782,8 @@ int p2r_forward_streamed(p2r_ctx* c, const void* host_trk_in, const void* host_h
     const UniformLaunch a{reinterpret_cast<const float*>(dt), reinterpret_cast<const float*>(dh),
                           reinterpret_cast<float*>(dout), nullptr, (uint32_t)(nb * bs), (uint32_t)bs, s->nlayer,
                           p.kfact / p.b_scale, p.options};
-    launch_uniform(semantics, p.precision == P2R_EXACT, c->tune.fwd_threads, c->tune.fwd_layers_per_launch, st, a);
+    launch_uniform(semantics, p.precision == P2R_EXACT, c->tune.fwd_threads, c->tune.fwd_layers_per_launch,
+                   c->tune.fwd_waves, st, a);
     const int rc = c->check_launch("p2r_forward_streamed");
     if (rc != P2R_OK) return rc;
     if (hipMemcpyAsync(static_cast<char*>(host_trk_out) + b0 * tw, dout, nb * tw, hipMemcpyDeviceToHost, st) !=

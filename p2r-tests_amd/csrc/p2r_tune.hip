@@ -88,7 +88,7 @@ struct Collect {
 
 p2r_tune_entry entry_of(const p2r_tuning& t, int bsize, float ms, int schedule) {
   return p2r_tune_entry{bsize, schedule == P2R_RAGGED_LAYER_ORDER ? t.ragged_threads : t.fwd_threads,
-                        t.fwd_layers_per_launch, schedule, t.checkpoint_stride, ms};
+                        t.fwd_layers_per_launch, schedule, t.checkpoint_stride, t.fwd_waves, ms};
 }
 
 // dst (dbs layout) <- src (sbs layout), one lane-word per thread; consecutive
@@ -128,6 +128,8 @@ int p2r_set_tuning(p2r_ctx* c, const p2r_tuning* in) {
   if (!schedule_ok(in->ragged_schedule))
     return c->fail(P2R_ERR_ARG, "ragged_schedule must be P2R_RAGGED_LAYER_ORDER, 1, 2 or 4");
   if (!stride_ok(in->checkpoint_stride)) return c->fail(P2R_ERR_ARG, "checkpoint_stride must be 1, 2, 4 or 8");
+  if (in->fwd_waves != 0 && in->fwd_waves != 3 && in->fwd_waves != 4)
+    return c->fail(P2R_ERR_ARG, "fwd_waves must be 0, 3 or 4");
   c->tune = *in;
   return P2R_OK;
 }
@@ -150,18 +152,37 @@ int p2r_autotune_forward(p2r_ctx* c, const void* trk_in, const void* hits, void*
       cand.push_back(t);
     }
   }
+  auto run = [&] { return p2r_forward(c, trk_in, hits, trk_out, s, semantics, params, nullptr, stream); };
   std::vector<float> ms(cand.size(), 0.f);
-  const int rc = time_candidates(
-      c, st, reps, (int)cand.size(), [&](int i) { c->tune = cand[i]; },
-      [&] { return p2r_forward(c, trk_in, hits, trk_out, s, semantics, params, nullptr, stream); }, ms.data());
+  int rc = time_candidates(c, st, reps, (int)cand.size(), [&](int i) { c->tune = cand[i]; }, run, ms.data());
+  if (rc != P2R_OK) {
+    c->tune = keep;
+    return rc;
+  }
+  // register budget at the best workgroup size / layer split
+  const size_t b0 = std::min_element(ms.begin(), ms.end()) - ms.begin();
+  for (int waves : {3, 4}) {
+    p2r_tuning t = cand[b0];
+    t.fwd_waves = waves;
+    cand.push_back(t);
+  }
+  float wms[2] = {};
+  rc = time_candidates(c, st, reps, 2, [&](int i) { c->tune = cand[cand.size() - 2 + i]; }, run, wms);
   c->tune = keep;
   if (rc != P2R_OK) return rc;
+  ms.push_back(wms[0]);
+  ms.push_back(wms[1]);
   Collect col;
-  for (size_t i = 0; i < cand.size(); ++i) col.add(entry_of(cand[i], s->bsize, ms[i], 0));
+  for (size_t i = 0; i < cand.size(); ++i) {
+    p2r_tune_entry e = entry_of(cand[i], s->bsize, ms[i], 0);
+    e.waves = cand[i].fwd_waves;
+    col.add(e);
+  }
   col.finish(report);
-  const p2r_tune_entry& b = col.rep.entry[col.rep.best];
-  c->tune.fwd_threads = b.threads;
-  c->tune.fwd_layers_per_launch = b.layers_per_launch;
+  const p2r_tuning& b = cand[col.rep.best];
+  c->tune.fwd_threads = b.fwd_threads;
+  c->tune.fwd_layers_per_launch = b.fwd_layers_per_launch;
+  c->tune.fwd_waves = b.fwd_waves;
   return P2R_OK;
 }
 

@@ -31,7 +31,7 @@ class p2r_gen_config(C.Structure):
 
 class p2r_tuning(C.Structure):
     _fields_ = [("fwd_threads", C.c_int32), ("fwd_layers_per_launch", C.c_int32), ("ragged_schedule", C.c_int32),
-                ("ragged_threads", C.c_int32), ("checkpoint_stride", C.c_int32)]
+                ("ragged_threads", C.c_int32), ("checkpoint_stride", C.c_int32), ("fwd_waves", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -39,7 +39,7 @@ class p2r_tuning(C.Structure):
 
 class p2r_tune_entry(C.Structure):
     _fields_ = [("bsize", C.c_int32), ("threads", C.c_int32), ("layers_per_launch", C.c_int32),
-                ("schedule", C.c_int32), ("checkpoint_stride", C.c_int32), ("ms", C.c_float)]
+                ("schedule", C.c_int32), ("checkpoint_stride", C.c_int32), ("waves", C.c_int32), ("ms", C.c_float)]
 
 
 class p2r_tune_report(C.Structure):

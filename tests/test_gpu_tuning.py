@@ -41,10 +41,11 @@ def test_tuning_roundtrip_and_validation(ctx):
     import p2r_amd
     t = ctx.get_tuning()
     assert t.as_dict() == {"fwd_threads": 256, "fwd_layers_per_launch": 0, "ragged_schedule": -1,
-                           "ragged_threads": 256, "checkpoint_stride": 4}
+                           "ragged_threads": 256, "checkpoint_stride": 4, "fwd_waves": 0}
     ctx.set_tuning(fwd_threads=128, checkpoint_stride=2)
     assert ctx.get_tuning().fwd_threads == 128 and ctx.get_tuning().checkpoint_stride == 2
-    for bad in ({"fwd_threads": 96}, {"ragged_schedule": 3}, {"checkpoint_stride": 3}, {"fwd_layers_per_launch": -1}):
+    for bad in ({"fwd_threads": 96}, {"ragged_schedule": 3}, {"checkpoint_stride": 3}, {"fwd_layers_per_launch": -1},
+                {"fwd_waves": 2}):
         with pytest.raises(p2r_amd.P2RError):
             ctx.set_tuning(**bad)
     assert ctx.get_tuning().fwd_threads == 128  # a rejected setting changes nothing
@@ -59,6 +60,14 @@ def test_launch_choices_are_bit_identical(ctx, batch, sem):
             ctx.set_tuning(fwd_threads=threads, fwd_layers_per_launch=per)
             o, c = _fwd(ctx, dt, dh, shape, sem)
             assert np.array_equal(o, ref_o) and np.array_equal(c, ref_c), (threads, per)
+    for waves in (3, 4):  # register budgets: spills change nothing but speed
+        ctx.set_tuning(fwd_threads=256, fwd_layers_per_launch=0, fwd_waves=waves)
+        for prec in ("exact", "fast"):
+            o, c = _fwd(ctx, dt, dh, shape, sem, prec)
+            ctx.set_tuning(fwd_waves=0)
+            o0, c0 = _fwd(ctx, dt, dh, shape, sem, prec)
+            ctx.set_tuning(fwd_waves=waves)
+            assert np.array_equal(o, o0) and np.array_equal(c, c0), (waves, prec)
 
 
 def test_autotune_forward_picks_a_candidate(ctx, batch):
@@ -71,9 +80,11 @@ def test_autotune_forward_picks_a_candidate(ctx, batch):
     best = ents[rep.best]
     assert best["ms"] == min(e["ms"] for e in ents)
     t = ctx.get_tuning()
-    assert (t.fwd_threads, t.fwd_layers_per_launch) == (best["threads"], best["layers_per_launch"])
+    assert (t.fwd_threads, t.fwd_layers_per_launch, t.fwd_waves) == (best["threads"], best["layers_per_launch"],
+                                                                      best["waves"])
+    assert sorted({e["waves"] for e in ents}) == [0, 3, 4]
     o, c = _fwd(ctx, dt, dh, shape, "reference")
-    ctx.set_tuning(fwd_threads=256, fwd_layers_per_launch=0)
+    ctx.set_tuning(fwd_threads=256, fwd_layers_per_launch=0, fwd_waves=0)
     o0, c0 = _fwd(ctx, dt, dh, shape, "reference")
     assert np.array_equal(o, o0) and np.array_equal(c, c0)
 
