@@ -221,6 +221,8 @@ def main():
         cpu = cpu_baseline(shape, trk, hits, args.semantics, args.cpu_seconds)
 
     achieved_tflops = value * FLOP_PER_UPDATE / 1e12
+    per = ctx.get_tuning().fwd_layers_per_launch
+    launches_per_step = 1 if per <= 0 or per >= shape.nlayer else -(-shape.nlayer // per)
     bytes_per_launch = shape.ntrk_total * (2 * TRK_BYTES + HIT_BYTES * shape.nlayer)
     traffic = measured_traffic()
     line = {
@@ -241,6 +243,7 @@ def main():
                    "nevts": args.nevts, "ntrks": args.ntrks, "nlayer": args.nlayer, "bsize": args.bsize,
                    "semantics": args.semantics, "precision": args.precision, "pass": "forward"},
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "launches_per_step": launches_per_step,
                      "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "flop_per_update": FLOP_PER_UPDATE,
                      "hbm_algorithmic_GBs": bytes_per_launch / (ms * 1e-3) / 1e9,
