@@ -131,7 +131,9 @@ int p2r_forward_ragged(p2r_ctx* ctx, const void* trk_in, const void* hits, void*
                        const p2r_params* params, float* chi2, int32_t tile_rounds, void* stream);
 
 /* Gradient precision of p2r_forward_adjoint.
- * P2R_GRAD_F32:   every layer in FP32 (the arithmetic of params->precision).
+ * The forward pass (trk_out, chi2) uses params->precision as p2r_forward does;
+ * the reverse sweep uses fused multiply-adds whatever that precision.
+ * P2R_GRAD_F32:   every layer in FP32.
  * P2R_GRAD_MIXED: the state carried from layer to layer (and the checkpoints)
  *                 in FP64, each layer's reverse sweep in FP32.
  * P2R_GRAD_F64:   everything in FP64. */

@@ -357,7 +357,10 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_kernel(
     float* __restrict__ gpar, float* __restrict__ ghit, float* __restrict__ gk, double* __restrict__ partials,
     RS* __restrict__ ckpt, uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
   using MS = MathOf<RS, EXACT>;
-  using ML = MathOf<RL, EXACT>;
+  // The forward pass and the segment recomputation run in the requested arithmetic (outputs and chi2 as
+  // p2r_forward's); the reverse sweep always in the fused instance: gradients have no reference rounding
+  // to reproduce, and they are checked against FP64 to a tolerance.
+  using ML = MathOf<RL, false>;
   constexpr int kWaves = kAdjThreads / 64;
   extern __shared__ double lds_global[];  // [kWaves][4*nlayer + 1] when partials != nullptr
   const uint32_t g = blockIdx.x * kAdjThreads + threadIdx.x;
