@@ -55,9 +55,11 @@ struct p2r_ctx {
     if (e != hipSuccess) return fail(P2R_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
     return P2R_OK;
   }
-  // Ensures at least `bytes` of workspace; returns nullptr on failure.
+  // Ensures at least `bytes` of workspace; returns nullptr on failure.  Growing it waits for the device
+  // first: kernels of earlier calls, on any stream, may still be reading the old buffer.
   void* workspace(size_t bytes) {
     if (bytes <= ws_bytes) return ws;
+    if (ws) (void)hipDeviceSynchronize();
     release_workspace();
     if (hipMalloc(&ws, bytes) != hipSuccess) {
       ws = nullptr;
