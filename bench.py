@@ -58,6 +58,8 @@ def main():
     ap.add_argument("--alignment-tracks", type=int, default=1000000)
     ap.add_argument("--no-parity", action="store_true", help="skip the parity spot check")
     ap.add_argument("--no-autotune", action="store_true", help="time the library's default launch choices")
+    ap.add_argument("--fwd-waves", type=int, default=None, choices=[0, 3, 4],
+                    help="set the forward kernels' register budget (waves per SIMD) instead of autotuning")
     ap.add_argument("--no-transfers", action="store_true", help="skip the host-to-host (PCIe-inclusive) timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (steps times) and exit: the "
@@ -103,7 +105,10 @@ def main():
 
     # launch choices for this shape, measured on these buffers (workgroup size x layers per launch)
     tuning = None
-    if not args.no_autotune:
+    if args.fwd_waves is not None:
+        ctx.set_tuning(fwd_waves=args.fwd_waves)
+        tuning = {"chosen": ctx.get_tuning().as_dict(), "candidates": None, "set_by": "--fwd-waves"}
+    elif not args.no_autotune:
         rep = ctx.autotune_forward(d_trk, d_hit, d_out, shape, args.semantics, precision=args.precision, reps=5,
                                    stream=stream)
         tuning = {"chosen": ctx.get_tuning().as_dict(), "candidates": rep.as_list()}
