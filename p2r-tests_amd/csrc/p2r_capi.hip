@@ -479,19 +479,30 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_kernel(
     for (int j = threadIdx.x; j < nglob; j += kAdjThreads) {
       double acc = 0.0;
       for (int w = 0; w < kWaves; ++w) acc += lds_global[w * nglob + j];
-      partials[(size_t)blockIdx.x * nglob + j] = acc;
+      partials[(size_t)j * gridDim.x + blockIdx.x] = acc;  // [global entry][workgroup]
     }
   }
 }
 
-// grad_global[j] = sum over workgroups of partials[b][j], in workgroup order.
-__global__ void finalize_global_kernel(const double* __restrict__ partials, double* __restrict__ out, int nblocks,
-                                       int nglob, double bscale_factor) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nglob) return;
+// grad_global[j] = sum over workgroups of partials[j][b]: one workgroup per entry j; thread t sums
+// b = t, t + 256, ... in order (coalesced rows), then a fixed pairwise tree in LDS.  The order depends
+// only on the number of workgroups, so the result is bit-reproducible.
+constexpr int kFinThreads = 256;
+__global__ __launch_bounds__(kFinThreads) void finalize_global_kernel(const double* __restrict__ partials,
+                                                                      double* __restrict__ out, int nblocks,
+                                                                      int nglob, double bscale_factor) {
+  __shared__ double red[kFinThreads];
+  const int j = blockIdx.x, t = threadIdx.x;
+  const double* row = partials + (size_t)j * nblocks;
   double acc = 0.0;
-  for (int b = 0; b < nblocks; ++b) acc += partials[(size_t)b * nglob + j];
-  out[j] = (j == nglob - 1) ? acc * bscale_factor : acc;
+  for (int b = t; b < nblocks; b += kFinThreads) acc += row[b];
+  red[t] = acc;
+  __syncthreads();
+  for (int h = kFinThreads / 2; h > 0; h >>= 1) {
+    if (t < h) red[t] += red[t + h];
+    __syncthreads();
+  }
+  if (t == 0) out[j] = (j == nglob - 1) ? red[0] * bscale_factor : red[0];
 }
 
 // Block-level propagateToR over nblk blocks with caller strides.
@@ -894,8 +905,8 @@ int p2r_forward_adjoint(p2r_ctx* c, const void* trk_in, const void* hits, void* 
   if (rc != P2R_OK || !grad_global) return rc;
   // d/d b_scale = (d/d kfact_eff) * (-kfact / b^2); the kernel summed -kf q dchi2/dk = -kf dchi2/dkfact_eff,
   // which equals b * d/db: divide by b.
-  finalize_global_kernel<<<(nglob + 63) / 64, 64, 0, st>>>(partials, grad_global, (int)grid, nglob,
-                                                           1.0 / (double)p.b_scale);
+  finalize_global_kernel<<<nglob, kFinThreads, 0, st>>>(partials, grad_global, (int)grid, nglob,
+                                                       1.0 / (double)p.b_scale);
   return c->check_launch("p2r_forward_adjoint (reduction)");
 }
 
