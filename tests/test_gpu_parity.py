@@ -375,3 +375,17 @@ def test_options_rejected_where_unsupported(ctx):
         ctx.forward_adjoint(dt, dh, torch.empty_like(dt), shape, params=p2r_amd.default_params("exact", options=1))
     with pytest.raises(p2r_amd.P2RError):
         ctx.forward(dt, dh, torch.empty_like(dt), shape, params=p2r_amd.default_params("exact", options=8))
+
+
+@pytest.mark.parametrize("semantics", ["reference", "chained"])
+def test_exact_full_headline_shape_matches_oracle(orc, ctx, semantics):
+    """The whole 100 events x 9600 tracks x 20 layers batch (19.2M updates) through the exact path with the
+    tuned register budget, every word compared with the oracle (run threaded on the host)."""
+    import p2r_amd
+    shape = Shape(100, 9600, 20, 32)
+    trk, hits = p2r_amd.gen_uniform(shape)
+    ctx.set_tuning(fwd_waves=3)
+    try:
+        check_exact(orc, ctx, trk, hits, shape, semantics, "full")
+    finally:
+        ctx.set_tuning(fwd_waves=0)
