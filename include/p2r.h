@@ -199,6 +199,31 @@ int p2r_forward_streamed(p2r_ctx* ctx, const void* host_trk_in, const void* host
                          void* dev_trk_in, void* dev_hits, void* dev_trk_out, const p2r_shape* shape, int semantics,
                          const p2r_params* params, int32_t nstreams);
 
+/* ---- HIP graphs ----------------------------------------------------------------
+ * One forward / uneven-batch / adjoint call, captured once with its arguments
+ * (pointers, shape, params, the context's tuning at capture time) into an
+ * instantiated HIP graph, then replayed with p2r_graph_launch on any stream.
+ * The enqueue paths issue kernel launches only (no allocation, no host
+ * synchronisation, no memset nodes); the adjoint's workspace is sized before its
+ * capture and must not be grown by a later call with a larger shape while the
+ * graph is in use (the context keeps the buffer; a larger adjoint call on the same
+ * context replaces it).  The ragged entry reads blk_offsets on the host at capture
+ * time.  Replays run concurrently with nothing else on the context's workspace.
+ * Not in the reference (its drivers launch directly, hip.cu:963-975). */
+typedef struct p2r_graph p2r_graph;
+int p2r_graph_forward(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* shape,
+                      int semantics, const p2r_params* params, float* chi2, p2r_graph** out);
+int p2r_graph_forward_ragged(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out,
+                             const int64_t* blk_offsets, int32_t nevts, const uint32_t* layer_mask, int32_t nlayer,
+                             int32_t bsize, int semantics, const p2r_params* params, float* chi2,
+                             int32_t tile_rounds, p2r_graph** out);
+int p2r_graph_forward_adjoint(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out,
+                              const p2r_shape* shape, const p2r_params* params, int grad_precision,
+                              int32_t checkpoint_stride, float* chi2, float* grad_par, float* grad_hits,
+                              float* grad_kfact, double* grad_global, p2r_graph** out);
+int p2r_graph_launch(p2r_graph* graph, void* stream);
+int p2r_graph_destroy(p2r_graph* graph);
+
 /* ---- Launch choices and the autotuner ---------------------------------------
  * The reference fixes Matriplex width, threads per block and layers per kernel
  * at compile time (build.py:68-80 sweeps them by rebuilding; hip_local.cu:911

@@ -19,6 +19,13 @@
 
 using namespace p2r;
 
+// An instantiated graph of one captured call (p2r_graph_* in include/p2r.h).
+struct p2r_graph {
+  int device = 0;
+  hipGraphExec_t exec = nullptr;  // null when the captured call launched nothing (empty batch)
+  void* ws = nullptr;             // workspace owned by the graph (adjoint), freed with it
+};
+
 namespace {
 
 constexpr int kThreads = 256;
@@ -674,6 +681,128 @@ void launch_adjoint_p(int gp, unsigned grid, size_t lds, hipStream_t st, const f
     launch_adjoint_t<double, double, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt, ntrk,
                                                 bs, nlayer, kf);
 }
+
+inline size_t ck_bytes_aligned(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Validated arguments and workspace layout of one adjoint call.
+struct AdjointPlan {
+  uint32_t n = 0, bs = 0;
+  int nlayer = 0, nglob = 0, cs = 0, grad_precision = 0;
+  unsigned grid = 0;
+  size_t ck_bytes = 0, ws_bytes = 0, lds = 0;
+  p2r_params p{};
+};
+
+int adjoint_plan(p2r_ctx* c, const void* trk_in, const void* hits, const void* trk_out, const p2r_shape* s,
+                 const p2r_params* params, int grad_precision, int32_t checkpoint_stride, bool global,
+                 AdjointPlan& pl) {
+  if (!trk_in || !hits || !trk_out || !s) return c->fail(P2R_ERR_ARG, "null pointer argument");
+  if (s->nevts < 0 || s->ntrks < 0 || s->nlayer < 1 || s->nlayer > 64 || s->bsize < 1 || s->bsize > 1024)
+    return c->fail(P2R_ERR_SHAPE, "bad shape nevts=%d ntrks=%d nlayer=%d bsize=%d", s->nevts, s->ntrks, s->nlayer,
+                   s->bsize);
+  if (s->ntrks % s->bsize != 0) return c->fail(P2R_ERR_SHAPE, "ntrks must be a multiple of bsize");
+  if (grad_precision < P2R_GRAD_F32 || grad_precision > P2R_GRAD_F64)
+    return c->fail(P2R_ERR_ARG, "bad grad_precision %d", grad_precision);
+  const int cs = checkpoint_stride ? checkpoint_stride : c->tune.checkpoint_stride;
+  if (cs != 1 && cs != 2 && cs != 4 && cs != 8)
+    return c->fail(P2R_ERR_ARG, "checkpoint_stride must be 0, 1, 2, 4 or 8");
+  const uint64_t ntrk = (uint64_t)s->nevts * (uint64_t)s->ntrks;
+  if (ntrk >= (1ull << 31)) return c->fail(P2R_ERR_SHAPE, "too many tracks");
+  const p2r_params p = params ? *params : p2r_default_params();
+  if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
+  if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
+  if (p.options != 0)
+    return c->fail(P2R_ERR_UNSUPPORTED, "p2r_forward_adjoint differentiates the reference's update only (options 0)");
+  pl.n = (uint32_t)ntrk;
+  pl.bs = (uint32_t)s->bsize;
+  pl.nlayer = s->nlayer;
+  pl.nglob = 4 * s->nlayer + 1;
+  pl.cs = cs;
+  pl.grad_precision = grad_precision;
+  pl.p = p;
+  if (ntrk == 0) return P2R_OK;
+  pl.grid = grid_for_n(ntrk, kAdjThreads);
+  const size_t sz = grad_precision == P2R_GRAD_F32 ? sizeof(float) : sizeof(double);
+  const size_t nck = (size_t)((pl.nlayer + cs - 1) / cs - 1);
+  pl.ck_bytes = std::max<size_t>(nck, 1) * 27 * ntrk * sz;
+  const size_t part_bytes = global ? (size_t)pl.grid * pl.nglob * sizeof(double) : 0;
+  pl.ws_bytes = ck_bytes_aligned(pl.ck_bytes) + part_bytes + 256;
+  pl.lds = global ? (size_t)(kAdjThreads / 64) * pl.nglob * sizeof(double) : 0;
+  return P2R_OK;
+}
+
+// Launches one adjoint call on `st` with the workspace `ws` (at least pl.ws_bytes).  Kernel launches only, so
+// the call can be captured into a graph.
+int adjoint_enqueue(p2r_ctx* c, const AdjointPlan& pl, const void* trk_in, const void* hits, void* trk_out,
+                    float* chi2, float* grad_par, float* grad_hits, float* grad_kfact, double* grad_global, char* ws,
+                    hipStream_t st) {
+  double* partials = grad_global ? reinterpret_cast<double*>(ws + ck_bytes_aligned(pl.ck_bytes)) : nullptr;
+  const float kf = pl.p.kfact / pl.p.b_scale;
+  auto T = static_cast<const float*>(trk_in);
+  auto H = static_cast<const float*>(hits);
+  auto O = static_cast<float*>(trk_out);
+  const int cs = pl.cs, gp = pl.grad_precision;
+  const bool exact = pl.p.precision == P2R_EXACT;
+#define P2R_ADJ(E, CS)                                                                                         \
+  launch_adjoint_p<E, CS>(gp, pl.grid, pl.lds, st, T, H, O, chi2, grad_par, grad_hits, grad_kfact, partials, ws, \
+                          pl.n, pl.bs, pl.nlayer, kf)
+  if (exact) {
+    if (cs == 1) P2R_ADJ(true, 1); else if (cs == 2) P2R_ADJ(true, 2); else if (cs == 4) P2R_ADJ(true, 4);
+    else P2R_ADJ(true, 8);
+  } else {
+    if (cs == 1) P2R_ADJ(false, 1); else if (cs == 2) P2R_ADJ(false, 2); else if (cs == 4) P2R_ADJ(false, 4);
+    else P2R_ADJ(false, 8);
+  }
+#undef P2R_ADJ
+  const int rc = c->check_launch("p2r_forward_adjoint");
+  if (rc != P2R_OK || !grad_global) return rc;
+  // d/d b_scale = (d/d kfact_eff) * (-kfact / b^2); the kernel summed -kf q dchi2/dk = -kf dchi2/dkfact_eff,
+  // which equals b * d/db: divide by b.
+  finalize_global_kernel<<<pl.nglob, kFinThreads, 0, st>>>(partials, grad_global, (int)pl.grid, pl.nglob,
+                                                          1.0 / (double)pl.p.b_scale);
+  return c->check_launch("p2r_forward_adjoint (reduction)");
+}
+
+// Captures what `enqueue(stream)` launches into an instantiated graph owned by a new p2r_graph (which takes
+// `ws` on success).  Relaxed capture mode: the enqueue paths query occupancy, which the stricter modes may
+// refuse while a capture is running.
+template <class F>
+int capture_graph(p2r_ctx* c, void* ws, p2r_graph** out, F&& enqueue) {
+  DeviceGuard g(c->device);
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+    return c->fail(P2R_ERR_HIP, "hipStreamCreateWithFlags failed");
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    (void)hipStreamDestroy(st);
+    return c->fail(P2R_ERR_HIP, "hipStreamBeginCapture failed");
+  }
+  const int rc = enqueue(st);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &graph);
+  (void)hipStreamDestroy(st);
+  if (rc != P2R_OK || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc != P2R_OK ? rc : c->fail(P2R_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  }
+  size_t nodes = 0;
+  if (hipGraphGetNodes(graph, nullptr, &nodes) != hipSuccess) nodes = 1;
+  hipGraphExec_t exec = nullptr;
+  if (nodes > 0 && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return c->fail(P2R_ERR_HIP, "hipGraphInstantiate failed");
+  }
+  (void)hipGraphDestroy(graph);
+  p2r_graph* gr = new (std::nothrow) p2r_graph();
+  if (!gr) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    return c->fail(P2R_ERR_NOMEM, "graph handle");
+  }
+  gr->device = c->device;
+  gr->exec = exec;
+  gr->ws = ws;
+  *out = gr;
+  return P2R_OK;
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -854,60 +983,86 @@ int p2r_forward_adjoint(p2r_ctx* c, const void* trk_in, const void* hits, void* 
                         const p2r_params* params, int grad_precision, int32_t checkpoint_stride, float* chi2,
                         float* grad_par, float* grad_hits, float* grad_kfact, double* grad_global, void* stream) {
   if (!c) return P2R_ERR_ARG;
-  if (!trk_in || !hits || !trk_out || !s) return c->fail(P2R_ERR_ARG, "null pointer argument");
-  if (s->nevts < 0 || s->ntrks < 0 || s->nlayer < 1 || s->nlayer > 64 || s->bsize < 1 || s->bsize > 1024)
-    return c->fail(P2R_ERR_SHAPE, "bad shape nevts=%d ntrks=%d nlayer=%d bsize=%d", s->nevts, s->ntrks, s->nlayer,
-                   s->bsize);
-  if (s->ntrks % s->bsize != 0) return c->fail(P2R_ERR_SHAPE, "ntrks must be a multiple of bsize");
-  if (grad_precision < P2R_GRAD_F32 || grad_precision > P2R_GRAD_F64)
-    return c->fail(P2R_ERR_ARG, "bad grad_precision %d", grad_precision);
-  const int cs = checkpoint_stride ? checkpoint_stride : c->tune.checkpoint_stride;
-  if (cs != 1 && cs != 2 && cs != 4 && cs != 8)
-    return c->fail(P2R_ERR_ARG, "checkpoint_stride must be 0, 1, 2, 4 or 8");
-  const uint64_t ntrk = (uint64_t)s->nevts * (uint64_t)s->ntrks;
-  if (ntrk >= (1ull << 31)) return c->fail(P2R_ERR_SHAPE, "too many tracks");
-  const p2r_params p = params ? *params : p2r_default_params();
-  if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
-  if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
-  if (p.options != 0)
-    return c->fail(P2R_ERR_UNSUPPORTED, "p2r_forward_adjoint differentiates the reference's update only (options 0)");
-  if (ntrk == 0) return P2R_OK;
+  AdjointPlan pl;
+  const int rc = adjoint_plan(c, trk_in, hits, trk_out, s, params, grad_precision, checkpoint_stride,
+                              grad_global != nullptr, pl);
+  if (rc != P2R_OK || pl.n == 0) return rc;
   DeviceGuard g(c->device);
-  const int nlayer = s->nlayer, nglob = 4 * nlayer + 1;
-  const unsigned grid = grid_for_n(ntrk, kAdjThreads);
-  const size_t sz = grad_precision == P2R_GRAD_F32 ? sizeof(float) : sizeof(double);
-  const size_t nck = (size_t)((nlayer + cs - 1) / cs - 1);
-  const size_t ck_bytes = std::max<size_t>(nck, 1) * 27 * ntrk * sz;
-  const size_t part_bytes = grad_global ? (size_t)grid * nglob * sizeof(double) : 0;
-  char* ws = static_cast<char*>(c->workspace(ck_bytes + part_bytes + 256));
-  if (!ws) return c->fail(P2R_ERR_NOMEM, "workspace of %zu bytes", ck_bytes + part_bytes);
-  double* partials = grad_global ? reinterpret_cast<double*>(ws + ((ck_bytes + 255) & ~(size_t)255)) : nullptr;
-  const size_t lds = grad_global ? (size_t)(kAdjThreads / 64) * nglob * sizeof(double) : 0;
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  const float kf = p.kfact / p.b_scale;
-  auto T = static_cast<const float*>(trk_in);
-  auto H = static_cast<const float*>(hits);
-  auto O = static_cast<float*>(trk_out);
-  const uint32_t n = (uint32_t)ntrk, bs = (uint32_t)s->bsize;
-  const bool exact = p.precision == P2R_EXACT;
-#define P2R_ADJ(E, CS)                                                                                           \
-  launch_adjoint_p<E, CS>(grad_precision, grid, lds, st, T, H, O, chi2, grad_par, grad_hits, grad_kfact, partials, ws, \
-                          n, bs, nlayer, kf)
-  if (exact) {
-    if (cs == 1) P2R_ADJ(true, 1); else if (cs == 2) P2R_ADJ(true, 2); else if (cs == 4) P2R_ADJ(true, 4);
-    else P2R_ADJ(true, 8);
-  } else {
-    if (cs == 1) P2R_ADJ(false, 1); else if (cs == 2) P2R_ADJ(false, 2); else if (cs == 4) P2R_ADJ(false, 4);
-    else P2R_ADJ(false, 8);
+  char* ws = static_cast<char*>(c->workspace(pl.ws_bytes));
+  if (!ws) return c->fail(P2R_ERR_NOMEM, "workspace of %zu bytes", pl.ws_bytes);
+  return adjoint_enqueue(c, pl, trk_in, hits, trk_out, chi2, grad_par, grad_hits, grad_kfact, grad_global, ws,
+                         static_cast<hipStream_t>(stream));
+}
+
+// ---- HIP graphs ------------------------------------------------------------------------------------------------
+int p2r_graph_forward(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* s,
+                      int semantics, const p2r_params* params, float* chi2, p2r_graph** out) {
+  if (!c) return P2R_ERR_ARG;
+  if (!out) return c->fail(P2R_ERR_ARG, "null graph pointer");
+  *out = nullptr;
+  return capture_graph(c, nullptr, out, [&](hipStream_t st) {
+    return p2r_forward(c, trk_in, hits, trk_out, s, semantics, params, chi2, st);
+  });
+}
+
+int p2r_graph_forward_ragged(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out,
+                             const int64_t* blk_offsets, int32_t nevts, const uint32_t* layer_mask, int32_t nlayer,
+                             int32_t bsize, int semantics, const p2r_params* params, float* chi2,
+                             int32_t tile_rounds, p2r_graph** out) {
+  if (!c) return P2R_ERR_ARG;
+  if (!out) return c->fail(P2R_ERR_ARG, "null graph pointer");
+  *out = nullptr;
+  return capture_graph(c, nullptr, out, [&](hipStream_t st) {
+    return p2r_forward_ragged(c, trk_in, hits, trk_out, blk_offsets, nevts, layer_mask, nlayer, bsize, semantics,
+                              params, chi2, tile_rounds, st);
+  });
+}
+
+int p2r_graph_forward_adjoint(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out,
+                              const p2r_shape* s, const p2r_params* params, int grad_precision,
+                              int32_t checkpoint_stride, float* chi2, float* grad_par, float* grad_hits,
+                              float* grad_kfact, double* grad_global, p2r_graph** out) {
+  if (!c) return P2R_ERR_ARG;
+  if (!out) return c->fail(P2R_ERR_ARG, "null graph pointer");
+  *out = nullptr;
+  AdjointPlan pl;
+  int rc = adjoint_plan(c, trk_in, hits, trk_out, s, params, grad_precision, checkpoint_stride,
+                        grad_global != nullptr, pl);
+  if (rc != P2R_OK) return rc;
+  // The graph owns its workspace (allocated here, before the capture), so later calls on the context, of any
+  // size, leave it alone.
+  DeviceGuard g(c->device);
+  void* ws = nullptr;
+  if (pl.n > 0 && hipMalloc(&ws, pl.ws_bytes) != hipSuccess)
+    return c->fail(P2R_ERR_NOMEM, "graph workspace of %zu bytes", pl.ws_bytes);
+  rc = capture_graph(c, ws, out, [&](hipStream_t st) {
+    if (pl.n == 0) return (int)P2R_OK;
+    return adjoint_enqueue(c, pl, trk_in, hits, trk_out, chi2, grad_par, grad_hits, grad_kfact, grad_global,
+                           static_cast<char*>(ws), st);
+  });
+  if (rc != P2R_OK && ws) (void)hipFree(ws);
+  return rc;
+}
+
+int p2r_graph_launch(p2r_graph* graph, void* stream) {
+  if (!graph) return P2R_ERR_ARG;
+  if (!graph->exec) return P2R_OK;  // captured an empty batch
+  DeviceGuard g(graph->device);
+  return hipGraphLaunch(graph->exec, static_cast<hipStream_t>(stream)) == hipSuccess ? P2R_OK : P2R_ERR_HIP;
+}
+
+int p2r_graph_destroy(p2r_graph* graph) {
+  if (!graph) return P2R_ERR_ARG;
+  {
+    DeviceGuard g(graph->device);
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    if (graph->ws) {
+      (void)hipDeviceSynchronize();  // replays still in flight may use the workspace
+      (void)hipFree(graph->ws);
+    }
   }
-#undef P2R_ADJ
-  int rc = c->check_launch("p2r_forward_adjoint");
-  if (rc != P2R_OK || !grad_global) return rc;
-  // d/d b_scale = (d/d kfact_eff) * (-kfact / b^2); the kernel summed -kf q dchi2/dk = -kf dchi2/dkfact_eff,
-  // which equals b * d/db: divide by b.
-  finalize_global_kernel<<<nglob, kFinThreads, 0, st>>>(partials, grad_global, (int)grid, nglob,
-                                                       1.0 / (double)p.b_scale);
-  return c->check_launch("p2r_forward_adjoint (reduction)");
+  delete graph;
+  return P2R_OK;
 }
 
 int p2r_propagateToR(p2r_ctx* c, const float* inErr, const float* inPar, const int32_t* inChg, const float* msP,

@@ -45,6 +45,30 @@ def default_params(precision="exact", b_scale=1.0, options=0):
     return p2r_params(p.kfact, float(b_scale), prec, int(options))
 
 
+class Graph:
+    """An instantiated HIP graph of one captured call (p2r_graph_launch / p2r_graph_destroy)."""
+
+    def __init__(self, handle, device):
+        self._h = handle
+        self.device = device
+
+    def launch(self, stream=None):
+        st = lib().p2r_graph_launch(self._h, _stream(stream))
+        if st != P2R_OK:
+            raise P2RError(st, "p2r_graph_launch")
+
+    def close(self):
+        if self._h:
+            lib().p2r_graph_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """A device binding (p2r_create / p2r_destroy)."""
 
@@ -116,6 +140,42 @@ class Context:
                                        int(checkpoint_stride), _ptr(chi2), _ptr(grad_par), _ptr(grad_hits),
                                        _ptr(grad_kfact), _ptr(grad_global), _stream(stream))
         self._check(st, "p2r_forward_adjoint")
+
+    # ---- HIP graphs of one call (include/p2r.h p2r_graph_*) ----
+    def graph_forward(self, trk, hits, out, shape, semantics="reference", params=None, chi2=None, precision=None):
+        """Captures one p2r_forward call (these buffers, this shape, the current tuning) into a Graph."""
+        sem = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
+        sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
+        p = params if params is not None else default_params(precision or "exact")
+        h = C.c_void_p()
+        self._check(lib().p2r_graph_forward(self._h, _ptr(trk), _ptr(hits), _ptr(out), C.byref(sh), sem, C.byref(p),
+                                            _ptr(chi2), C.byref(h)), "p2r_graph_forward")
+        return Graph(h, self.device)
+
+    def graph_forward_ragged(self, trk, hits, out, blk_offsets, layer_mask, nlayer, bsize, semantics="chained",
+                             params=None, chi2=None, precision=None, tile_rounds=0):
+        sem = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
+        p = params if params is not None else default_params(precision or "exact")
+        offs = np.ascontiguousarray(blk_offsets, np.int64)
+        h = C.c_void_p()
+        self._check(lib().p2r_graph_forward_ragged(self._h, _ptr(trk), _ptr(hits), _ptr(out), _ptr(offs),
+                                                   len(offs) - 1, _ptr(layer_mask), int(nlayer), int(bsize), sem,
+                                                   C.byref(p), _ptr(chi2), int(tile_rounds), C.byref(h)),
+                    "p2r_graph_forward_ragged")
+        return Graph(h, self.device)
+
+    def graph_forward_adjoint(self, trk, hits, out, shape, params=None, grad_precision="f32", checkpoint_stride=0,
+                              chi2=None, grad_par=None, grad_hits=None, grad_kfact=None, grad_global=None,
+                              precision=None):
+        sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
+        p = params if params is not None else default_params(precision or "exact")
+        gp = GRAD_PRECISION[grad_precision] if isinstance(grad_precision, str) else int(grad_precision)
+        h = C.c_void_p()
+        self._check(lib().p2r_graph_forward_adjoint(self._h, _ptr(trk), _ptr(hits), _ptr(out), C.byref(sh),
+                                                    C.byref(p), gp, int(checkpoint_stride), _ptr(chi2),
+                                                    _ptr(grad_par), _ptr(grad_hits), _ptr(grad_kfact),
+                                                    _ptr(grad_global), C.byref(h)), "p2r_graph_forward_adjoint")
+        return Graph(h, self.device)
 
     # ---- launch choices and autotuners (include/p2r.h p2r_tuning, p2r_autotune_*) ----
     def get_tuning(self):

@@ -92,6 +92,14 @@ def lib():
             "p2r_autotune_adjoint": ([vp, vp, vp, vp, C.POINTER(p2r_shape), C.POINTER(p2r_params), C.c_int, vp, vp,
                                       vp, vp, vp, i32, C.POINTER(p2r_tune_report), vp], C.c_int),
             "p2r_repack": ([vp, vp, i32, vp, i32, i32, i32, i32, i32, vp], C.c_int),
+            "p2r_graph_forward": ([vp, vp, vp, vp, C.POINTER(p2r_shape), C.c_int, C.POINTER(p2r_params), vp,
+                                   C.POINTER(vp)], C.c_int),
+            "p2r_graph_forward_ragged": ([vp, vp, vp, vp, vp, i32, vp, i32, i32, C.c_int, C.POINTER(p2r_params), vp,
+                                          i32, C.POINTER(vp)], C.c_int),
+            "p2r_graph_forward_adjoint": ([vp, vp, vp, vp, C.POINTER(p2r_shape), C.POINTER(p2r_params), C.c_int, i32,
+                                           vp, vp, vp, vp, vp, C.POINTER(vp)], C.c_int),
+            "p2r_graph_launch": ([vp, vp], C.c_int),
+            "p2r_graph_destroy": ([vp], C.c_int),
             "p2r_autotune_width": ([vp, vp, vp, C.POINTER(p2r_shape), C.c_int, C.POINTER(p2r_params), vp, i32, i32,
                                     C.POINTER(p2r_tune_report), vp], C.c_int),
         }
