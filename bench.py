@@ -364,7 +364,8 @@ def ragged_bench(ctx, args, uniform_chained, headline):
     [500, 40000]), hit count per track uniform in 5..20 (random subset of the 20 layers), chained
     semantics, one launch replayed from a captured HIP graph.  For each Matriplex width 16/32/64 (inputs
     generated in that width) the autotuner times the schedules (layer order at 64/128/256 lanes per
-    workgroup, hit-count-sorted tiles of 1/2/4 rounds); the chosen one is then captured and replayed.
+    workgroup, the LDS pair stream, hit-count-sorted tiles of 1/2/4 rounds); the chosen one is then
+    captured and replayed.
     Rate = present-layer updates/s, compared with the uniform batch of the headline shape, chained."""
     import numpy as np
     import torch
@@ -395,13 +396,17 @@ def ragged_bench(ctx, args, uniform_chained, headline):
                     ctx.forward_ragged(dt, dh, do, offs, dm, 20, bs, "chained", precision=prec, stream=stream)
                 g.replay()
                 stream.synchronize()
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record(stream)
-                for _ in range(5):
-                    g.replay()
-                e1.record(stream)
+                rounds = []
+                for _ in range(3):  # median of 3 rounds of 10 replays
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(stream)
+                    for _ in range(10):
+                        g.replay()
+                    e1.record(stream)
+                    stream.synchronize()
+                    rounds.append(e0.elapsed_time(e1) / 10)
             stream.synchronize()
-            ms = e0.elapsed_time(e1) / 5
+            ms = sorted(rounds)[1]
             rate = updates / (ms * 1e-3)
             res["widths"][f"{prec}_bs{bs}"] = {
                 "graph_ms": ms, "updates_per_s": rate, "slots": int(mask.size), "gen_s": gen_s,

@@ -121,11 +121,15 @@ int p2r_forward(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_ou
  *     tracks are ordered by hit count before they are dealt to wavefronts
  *     (no idle lanes; each lane reads its own layers, so hit loads gather
  *     from many cache lines);
- *   P2R_RAGGED_AUTO (0): the library's choice (currently the layer order).
+ *   P2R_RAGGED_PAIR_STREAM (-2): each wavefront keeps the state of a tile of
+ *     128 slots in LDS and deals the tile's present (layer, slot) pairs, in
+ *     layer-major order, 64 per step to its lanes (no idle lanes; the hit rows
+ *     of a step belong to at most two layers, so loads stay near-coalesced);
+ *   P2R_RAGGED_AUTO (0): the library's choice (the context's ragged_schedule).
  * The reference has no uneven batches
  * (every event has ntrks tracks and every track nlayer hits, tbb.cpp:15-34);
  * with all masks full and equal event sizes the result equals p2r_forward's. */
-enum { P2R_RAGGED_AUTO = 0, P2R_RAGGED_LAYER_ORDER = -1 };
+enum { P2R_RAGGED_AUTO = 0, P2R_RAGGED_LAYER_ORDER = -1, P2R_RAGGED_PAIR_STREAM = -2 };
 int p2r_forward_ragged(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out, const int64_t* blk_offsets,
                        int32_t nevts, const uint32_t* layer_mask, int32_t nlayer, int32_t bsize, int semantics,
                        const p2r_params* params, float* chi2, int32_t tile_rounds, void* stream);
@@ -233,7 +237,8 @@ typedef struct p2r_tuning {
   int32_t fwd_threads;           /* workgroup size of p2r_forward: 64, 128 or 256 (default 256) */
   int32_t fwd_layers_per_launch; /* 0 = all layers in one launch (default); n = launches of n layers,
                                     the state passing through trk_out in between */
-  int32_t ragged_schedule;       /* what P2R_RAGGED_AUTO runs: P2R_RAGGED_LAYER_ORDER (default), 1, 2, 4 */
+  int32_t ragged_schedule;       /* what P2R_RAGGED_AUTO runs: P2R_RAGGED_PAIR_STREAM (default),
+                                    P2R_RAGGED_LAYER_ORDER, 1, 2, 4 */
   int32_t ragged_threads;        /* workgroup size of the layer-ordered schedule: 64, 128 or 256 (256) */
   int32_t checkpoint_stride;     /* what checkpoint_stride = 0 means in p2r_forward_adjoint: 1, 2, 4, 8 (4) */
   int32_t fwd_waves;             /* p2r_forward kernels compiled for at least this many waves per SIMD:
@@ -263,7 +268,7 @@ typedef struct p2r_tune_report {
  *            ceil(nlayer/2), ceil(nlayer/4)}, then fwd_waves {0, 3, 4} at the
  *            best of those;
  *   ragged:  ragged_schedule {layer order x ragged_threads {64, 128, 256},
- *            sorted tiles of 1, 2, 4 rounds};
+ *            pair stream, sorted tiles of 1, 2, 4 rounds};
  *   adjoint: checkpoint_stride {1, 2, 4, 8} (grad_precision as given). */
 int p2r_autotune_forward(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* shape,
                          int semantics, const p2r_params* params, int32_t reps, p2r_tune_report* report,

@@ -310,6 +310,172 @@ __global__ __launch_bounds__(kThreads) void fwd_ragged_layer_kernel(const float*
 }
 
 // ---------------------------------------------------------------------------
+// Uneven batch, pair-stream schedule.  Each wavefront owns a tile of kPsTile
+// slots whose state (27 floats, charge, chi2 running sum) lives in LDS for the
+// whole pass.  The tile's present (layer, slot) pairs are taken in layer-major
+// order, 64 at a time, and dealt to the lanes: no lane idles on an absent
+// layer, and the lanes of one step read the hit rows of at most two layers of
+// the tile's blocks (near-coalesced).  A step takes its pairs from a window of
+// kPsTile consecutive (layer, slot) positions, so no slot occurs twice in one
+// step; steps run in order within the wavefront, so a track's layers are
+// updated in increasing order exactly as in the one-lane-per-track kernels
+// (same per-lane arithmetic, same chi2 summation order: bit-identical).
+// Wavefronts are independent (no workgroup barrier); LDS is private per
+// wavefront.
+constexpr int kPsTile = 128;   // slots per wavefront tile (2 per lane)
+constexpr int kPsWords = 29;   // per slot: 27 state floats, charge, chi2 sum (when returned)
+constexpr int kPsWaves = 1;    // wavefronts per workgroup (one: 14.6 KB of LDS each, 11 per CU)
+constexpr uint32_t kPsNone = 0xffffffffu;
+
+// Orders this wavefront's LDS accesses across lanes (LDS operations of one
+// wavefront complete in issue order; this keeps the compiler from reordering).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One step of the schedule from window start `c` (position = layer*kPsTile + slot).
+// Returns this lane's pair code (slot | layer << 7 | last-layer flag << 12 |
+// lane-in-block << 13 | block - the tile's first block << 23) or kPsNone, and
+// the next window start in `next` (wave-uniform).  m, rb, ln: mask, relative
+// block and lane-in-block of this lane's two slots.
+__device__ __forceinline__ uint32_t ps_step(uint32_t c, const uint32_t m[2], const uint32_t rb[2], const uint32_t ln[2],
+                                           int lane, uint32_t* tbl, uint32_t& next) {
+  const uint32_t L0 = c >> 7, s0 = c & (kPsTile - 1);
+  uint64_t p1[2], p2[2];
+  bool bit[2], first[2];
+  int L[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const uint32_t slot = (uint32_t)lane + 64u * r;
+    first[r] = slot >= s0;
+    L[r] = first[r] ? (int)L0 : (int)L0 + 1;
+    bit[r] = L[r] < 32 && ((m[r] >> L[r]) & 1u);
+    p1[r] = __ballot(bit[r] && first[r]);
+    p2[r] = __ballot(bit[r] && !first[r]);
+  }
+  const uint32_t n1a = __popcll(p1[0]), n1 = n1a + __popcll(p1[1]);
+  const uint32_t n2a = __popcll(p2[0]), total = n1 + n2a + __popcll(p2[1]);
+  const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    if (!bit[r]) continue;
+    const uint32_t rank = first[r] ? (r ? n1a : 0u) + __popcll(p1[r] & below)
+                                   : n1 + (r ? n2a : 0u) + __popcll(p2[r] & below);
+    if (rank <= 64) {
+      const uint32_t last = (m[r] >> L[r]) <= 1u;  // no present layer above L
+      tbl[rank] = ((uint32_t)lane + 64u * r) | ((uint32_t)L[r] << 7) | (last << 12) | (ln[r] << 13) | (rb[r] << 23);
+    }
+  }
+  wave_lds_sync();
+  const uint32_t take = total < 64 ? total : 64u;
+  const uint32_t code = (uint32_t)lane < take ? tbl[lane] : kPsNone;
+  next = total > 64 ? (__builtin_amdgcn_readfirstlane(tbl[64]) & 0xfffu) : c + kPsTile;
+  wave_lds_sync();
+  return code;
+}
+
+template <int SEM, bool CHI2, bool EXACT>
+__global__ __launch_bounds__(64 * kPsWaves) void fwd_ragged_stream_kernel(const float* __restrict__ trk,
+                                                                           const float* __restrict__ hits,
+                                                                           float* __restrict__ out,
+                                                                           float* __restrict__ chi2,
+                                                                           const uint32_t* __restrict__ mask,
+                                                                           uint32_t nslot, uint32_t bs, int nlayer,
+                                                                           float kfact, int opts) {
+  constexpr int kW = CHI2 ? kPsWords : kPsWords - 1;  // the chi2 word only when it is returned
+  __shared__ float st_all[kPsWaves][kW][kPsTile];
+  __shared__ uint32_t tbl_all[kPsWaves][65];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t base = (blockIdx.x * kPsWaves + wv) * (uint32_t)kPsTile;
+  if (base >= nslot) return;  // whole wavefront; no workgroup barriers below
+  float(*st)[kPsTile] = st_all[wv];
+  uint32_t* tbl = tbl_all[wv];
+  const uint32_t layer_bits = nlayer >= 32 ? ~0u : ((1u << nlayer) - 1u);
+  const uint32_t blk0 = base / bs;  // the tile's blocks are blk0 .. blk0 + 127 at most
+  uint32_t m[2], b[2], rb[2], ln[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const uint32_t slot = (uint32_t)lane + 64u * r, g = base + slot;
+    m[r] = 0u;
+    b[r] = g / bs;
+    rb[r] = b[r] - blk0;
+    ln[r] = g - b[r] * bs;
+    if (g < nslot) {
+      const float* tb = trk + (size_t)b[r] * 28 * bs + ln[r];
+#pragma unroll
+      for (int f = 0; f < 28; ++f) st[f][slot] = tb[(size_t)f * bs];
+      if (CHI2) st[kW - 1][slot] = 0.f;
+      m[r] = mask[g] & layer_bits;
+    }
+  }
+  wave_lds_sync();
+  const size_t hstride = (size_t)9 * bs;
+  auto hit_ptr = [&](uint32_t code) {
+    const uint32_t L = (code >> 7) & 31u;
+    return hits + ((size_t)(blk0 + (code >> 23)) * nlayer + L) * hstride + ((code >> 13) & 1023u);
+  };
+  const uint32_t cend = (uint32_t)nlayer * kPsTile;
+  uint32_t c = 0, cn = 0;
+  uint32_t code_n = ps_step(c, m, rb, ln, lane, tbl, cn);
+  Hit hn;
+  if (code_n != kPsNone) load_hit(hit_ptr(code_n), bs, hn);
+  while (c < cend) {
+    const uint32_t code = code_n;
+    const Hit h = hn;
+    c = cn;
+    if (c < cend) {  // next step's pairs and hit loads, issued before this step's arithmetic
+      code_n = ps_step(c, m, rb, ln, lane, tbl, cn);
+      if (code_n != kPsNone) load_hit(hit_ptr(code_n), bs, hn);
+    }
+    if (code != kPsNone) {
+      const uint32_t slot = code & (kPsTile - 1);
+      Trk src;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) src.par[f] = st[f][slot];
+#pragma unroll
+      for (int f = 0; f < 21; ++f) src.cov[f] = st[6 + f][slot];
+      int q = __float_as_int(st[27][slot]);
+      float k = (float)q * kfact;
+      if (SEM == P2R_REFERENCE) opaque(src);
+      Trk prop;
+      Lane<EXACT>::propagate(src, k, h.pos[0], h.pos[1], prop);
+      float cc = Lane<EXACT>::update(prop, h);
+      if (opts) apply_options<EXACT>(opts, prop, q, k, kfact);
+      if (SEM == P2R_REFERENCE) {
+        opaque(prop);
+        opaque(cc);
+      }
+      if (CHI2) st[kW - 1][slot] += cc;
+      if (SEM == P2R_CHAINED) {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) st[f][slot] = prop.par[f];
+#pragma unroll
+        for (int f = 0; f < 21; ++f) st[6 + f][slot] = prop.cov[f];
+        st[27][slot] = __int_as_float(q);
+      } else if ((code >> 12) & 1u) {  // reference semantics: the last present layer's result is the output
+        float* ob = out + (size_t)(blk0 + (code >> 23)) * 28 * bs + ((code >> 13) & 1023u);
+        store_trk(ob, bs, prop);
+        reinterpret_cast<int*>(ob)[(size_t)27 * bs] = q;
+      }
+    }
+    wave_lds_sync();
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const uint32_t slot = (uint32_t)lane + 64u * r, g = base + slot;
+    if (g >= nslot) continue;
+    if (SEM == P2R_CHAINED || m[r] == 0u) {  // chained result, or a track with no present layer copied through
+      float* ob = out + (size_t)b[r] * 28 * bs + ln[r];
+#pragma unroll
+      for (int f = 0; f < 28; ++f) ob[(size_t)f * bs] = st[f][slot];
+    }
+    if (CHI2) chi2[g] = st[kW - 1][slot];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Forward + adjoint of the chained fit.  One lane per track.  Forward pass in
 // the state precision RS, checkpoints every CS layers (checkpoint 0 is the
 // input track, not stored); reverse sweep per segment: the segment's layer
@@ -637,6 +803,12 @@ int launch_ragged_t(p2r_ctx* c, hipStream_t st, const float* T, const float* H, 
 template <int SEM, bool CHI2, bool EXACT>
 int launch_ragged_r(int rounds, p2r_ctx* c, hipStream_t st, const float* T, const float* H, float* O, float* chi2,
                     const uint32_t* M, uint32_t nslot, uint32_t bs, int nlayer, float kf, int opts) {
+  if (rounds == P2R_RAGGED_PAIR_STREAM) {
+    const uint32_t ntiles = (nslot + kPsTile - 1) / kPsTile;
+    fwd_ragged_stream_kernel<SEM, CHI2, EXACT><<<(ntiles + kPsWaves - 1) / kPsWaves, 64 * kPsWaves, 0, st>>>(
+        T, H, O, chi2, M, nslot, bs, nlayer, kf, opts);
+    return P2R_OK;
+  }
   if (rounds == P2R_RAGGED_LAYER_ORDER) {
     const int threads = c->tune.ragged_threads;
     fwd_ragged_layer_kernel<SEM, CHI2, EXACT><<<grid_for_n(nslot, threads), threads, 0, st>>>(T, H, O, chi2, M, nslot,
@@ -948,9 +1120,10 @@ int p2r_forward_ragged(p2r_ctx* c, const void* trk_in, const void* hits, void* t
   if (nevts < 0 || nlayer < 1 || nlayer > 32 || bsize < 1 || bsize > 1024)
     return c->fail(P2R_ERR_SHAPE, "bad shape nevts=%d nlayer=%d bsize=%d (nlayer <= 32 for the 32-bit mask)", nevts,
                    nlayer, bsize);
-  if (tile_rounds != P2R_RAGGED_AUTO && tile_rounds != P2R_RAGGED_LAYER_ORDER && tile_rounds != 1 &&
-      tile_rounds != 2 && tile_rounds != 4)
-    return c->fail(P2R_ERR_ARG, "tile_rounds must be P2R_RAGGED_AUTO, P2R_RAGGED_LAYER_ORDER, 1, 2 or 4");
+  if (tile_rounds != P2R_RAGGED_AUTO && tile_rounds != P2R_RAGGED_LAYER_ORDER &&
+      tile_rounds != P2R_RAGGED_PAIR_STREAM && tile_rounds != 1 && tile_rounds != 2 && tile_rounds != 4)
+    return c->fail(P2R_ERR_ARG,
+                   "tile_rounds must be P2R_RAGGED_AUTO, P2R_RAGGED_LAYER_ORDER, P2R_RAGGED_PAIR_STREAM, 1, 2 or 4");
   if (blk_offsets[0] != 0) return c->fail(P2R_ERR_SHAPE, "blk_offsets[0] must be 0");
   for (int32_t e = 0; e < nevts; ++e)
     if (blk_offsets[e + 1] < blk_offsets[e]) return c->fail(P2R_ERR_SHAPE, "blk_offsets decrease at event %d", e);

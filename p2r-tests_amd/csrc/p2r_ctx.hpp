@@ -25,7 +25,7 @@ struct p2r_ctx {
   int num_cus = 0;
   char msg[256] = "no error";
   // Launch choices (p2r_set_tuning / the autotuners); defaults below.
-  p2r_tuning tune = {256, 0, P2R_RAGGED_LAYER_ORDER, 256, 4, 0};
+  p2r_tuning tune = {256, 0, P2R_RAGGED_PAIR_STREAM, 256, 4, 0};
   // Streams of p2r_forward_streamed, created on first use (default priority).
   hipStream_t streams[16] = {};
   hipStream_t stream(int i) {

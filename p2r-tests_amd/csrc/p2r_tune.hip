@@ -16,7 +16,9 @@
 namespace {
 
 bool threads_ok(int t) { return t == 64 || t == 128 || t == 256; }
-bool schedule_ok(int s) { return s == P2R_RAGGED_LAYER_ORDER || s == 1 || s == 2 || s == 4; }
+bool schedule_ok(int s) {
+  return s == P2R_RAGGED_LAYER_ORDER || s == P2R_RAGGED_PAIR_STREAM || s == 1 || s == 2 || s == 4;
+}
 bool stride_ok(int s) { return s == 1 || s == 2 || s == 4 || s == 8; }
 
 // Mean time (ms) of `reps` calls after 2 untimed ones; P2R status of the first failing call otherwise.
@@ -126,7 +128,7 @@ int p2r_set_tuning(p2r_ctx* c, const p2r_tuning* in) {
     return c->fail(P2R_ERR_ARG, "threads must be 64, 128 or 256");
   if (in->fwd_layers_per_launch < 0) return c->fail(P2R_ERR_ARG, "fwd_layers_per_launch must be >= 0");
   if (!schedule_ok(in->ragged_schedule))
-    return c->fail(P2R_ERR_ARG, "ragged_schedule must be P2R_RAGGED_LAYER_ORDER, 1, 2 or 4");
+    return c->fail(P2R_ERR_ARG, "ragged_schedule must be P2R_RAGGED_LAYER_ORDER, P2R_RAGGED_PAIR_STREAM, 1, 2 or 4");
   if (!stride_ok(in->checkpoint_stride)) return c->fail(P2R_ERR_ARG, "checkpoint_stride must be 1, 2, 4 or 8");
   if (in->fwd_waves != 0 && in->fwd_waves != 3 && in->fwd_waves != 4)
     return c->fail(P2R_ERR_ARG, "fwd_waves must be 0, 3 or 4");
@@ -197,7 +199,7 @@ int p2r_autotune_ragged(p2r_ctx* c, const void* trk_in, const void* hits, void* 
     int schedule, threads;
   };
   const Cand cand[] = {{P2R_RAGGED_LAYER_ORDER, 64}, {P2R_RAGGED_LAYER_ORDER, 128}, {P2R_RAGGED_LAYER_ORDER, 256},
-                       {1, 256}, {2, 256}, {4, 256}};
+                       {P2R_RAGGED_PAIR_STREAM, 128}, {1, 256}, {2, 256}, {4, 256}};
   constexpr int n = sizeof(cand) / sizeof(cand[0]);
   float ms[n] = {};
   const int rc = time_candidates(

@@ -251,7 +251,7 @@ def run_ragged(ctx, trk, hits, mask, offs, nlayer, bsize, semantics, precision="
     return do.cpu().numpy(), dc.cpu().numpy()
 
 
-@pytest.mark.parametrize("rounds", [-1, 1, 2, 4])
+@pytest.mark.parametrize("rounds", [-1, -2, 1, 2, 4])
 @pytest.mark.parametrize("semantics", ["chained", "reference"])
 def test_exact_ragged_matches_oracle(orc, ctx, semantics, rounds):
     """Uneven events + masked layers, one persistent launch: bit-identical to the oracle's masked run."""
@@ -268,7 +268,7 @@ def test_exact_ragged_matches_oracle(orc, ctx, semantics, rounds):
     assert n == 0 and nq == 0 and nc == 0
 
 
-@pytest.mark.parametrize("rounds", [-1, 1])
+@pytest.mark.parametrize("rounds", [-1, -2, 1])
 @pytest.mark.parametrize("bsize", [1, 8, 64])
 def test_exact_ragged_other_widths(orc, ctx, bsize, rounds):
     """Matriplex widths other than 32 through both schedules (chained, chi2 on)."""
@@ -289,17 +289,35 @@ def test_ragged_full_masks_equal_uniform(ctx):
     mask = np.full(shape.ntrk_total, (1 << 20) - 1, np.uint32)
     for sem in ("reference", "chained"):
         uni, _ = run_gpu(ctx, trk, hits, shape, sem)
-        rag, _ = run_ragged(ctx, trk, hits, mask, offs, 20, 32, sem)
-        assert bit_mismatches(uni, rag) == 0
+        for rounds in (-1, -2):
+            rag, _ = run_ragged(ctx, trk, hits, mask, offs, 20, 32, sem, rounds=rounds)
+            assert bit_mismatches(uni, rag) == 0, (sem, rounds)
 
 
-@pytest.mark.parametrize("rounds", [-1, 1])
+@pytest.mark.parametrize("rounds", [-1, -2, 1])
 def test_ragged_ignores_mask_bits_beyond_nlayer(orc, ctx, rounds):
     counts, trk, hits, mask, offs = _ragged_inputs(nevts=8, nlayer=12)
     junk = mask | np.uint32(0xFFFFF000)  # bits 12..31 name no hit block
     gpu, _ = run_ragged(ctx, trk, hits, junk, offs, 12, 32, "chained", rounds=rounds)
     cpu = orc.run_f32(trk, hits, int(offs[-1]), 12, 32, semantics=1, mask=mask, cr_trig=True)
     assert bit_mismatches(gpu.reshape(-1, 28, 32)[:, :27], cpu.reshape(-1, 28, 32)[:, :27]) == 0
+
+
+@pytest.mark.parametrize("nlayer,density", [(1, 0.5), (5, 0.1), (20, 0.03), (20, 0.97), (32, 0.5), (32, 0.0)])
+@pytest.mark.parametrize("semantics", ["chained", "reference"])
+def test_exact_pair_stream_mask_densities(orc, ctx, semantics, nlayer, density):
+    """The pair-stream schedule over masks from empty to full (windows with no pair, steps short of 64 pairs,
+    steps spanning two layers), 32-bit masks, and a tile cut by the end of the batch (13 blocks of 8 slots)."""
+    counts = np.array([40, 17, 0, 33], np.int32)
+    import p2r_amd
+    trk, hits, _, offs = p2r_amd.gen_ragged(counts, nlayer, 8, kmin=1, kmax=nlayer)
+    rng = np.random.default_rng(nlayer * 100 + int(density * 100))
+    bits = rng.random((offs[-1] * 8, nlayer)) < density
+    mask = (bits * (1 << np.arange(nlayer, dtype=np.uint64))).sum(1).astype(np.uint32)
+    sem = 0 if semantics == "reference" else 1
+    gpu, c2g = run_ragged(ctx, trk, hits, mask, offs, nlayer, 8, semantics, rounds=-2)
+    cpu, c2c = orc.run_f32(trk, hits, int(offs[-1]), nlayer, 8, semantics=sem, chi2=True, mask=mask, cr_trig=True)
+    assert bit_mismatches(gpu, cpu) == 0 and bit_mismatches(c2g, c2c) == 0
 
 
 def test_ragged_graph_replays_match_eager(ctx):
@@ -350,7 +368,7 @@ def test_exact_options_match_oracle(orc, ctx, semantics, options):
     assert bit_mismatches(base, cpu) > 0  # the options changed something on these inputs
 
 
-@pytest.mark.parametrize("rounds", [-1, 1])
+@pytest.mark.parametrize("rounds", [-1, -2, 1])
 def test_exact_options_ragged_match_oracle(orc, ctx, rounds):
     import p2r_amd
     counts, trk, hits, mask, offs = _ragged_inputs(nevts=10)

@@ -40,7 +40,7 @@ def batch():
 def test_tuning_roundtrip_and_validation(ctx):
     import p2r_amd
     t = ctx.get_tuning()
-    assert t.as_dict() == {"fwd_threads": 256, "fwd_layers_per_launch": 0, "ragged_schedule": -1,
+    assert t.as_dict() == {"fwd_threads": 256, "fwd_layers_per_launch": 0, "ragged_schedule": -2,
                            "ragged_threads": 256, "checkpoint_stride": 4, "fwd_waves": 0}
     ctx.set_tuning(fwd_threads=128, checkpoint_stride=2)
     assert ctx.get_tuning().fwd_threads == 128 and ctx.get_tuning().checkpoint_stride == 2
@@ -140,7 +140,8 @@ def test_autotune_ragged_and_adjoint(ctx):
     ref = do.cpu().numpy().view(np.uint32).copy()
     rep = ctx.autotune_ragged(dt, dh, do, offs, dm, 20, 32, "chained", reps=2)
     ents = rep.as_list()
-    assert rep.count == 6 and ctx.get_tuning().ragged_schedule == ents[rep.best]["schedule"]
+    assert rep.count == 7 and ctx.get_tuning().ragged_schedule == ents[rep.best]["schedule"]
+    assert -2 in [e["schedule"] for e in ents]
     for threads in (64, 128):
         ctx.set_tuning(ragged_schedule=-1, ragged_threads=threads)
         ctx.forward_ragged(dt, dh, do, offs, dm, 20, 32, "chained")
