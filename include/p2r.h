@@ -81,8 +81,18 @@ typedef struct p2r_params {
  *   [-pi, pi) up to FP32 rounding (the multiply by 1/2pi is an FP32 constant).
  * P2R_OPT_CHARGE_FROM_IPT: an update that makes 1/pt negative flips the charge
  *   and 1/pt's sign; the next layer propagates with the new charge, and the
- *   output block carries it. */
-typedef enum p2r_option { P2R_OPT_WRAP_PHI = 1, P2R_OPT_CHARGE_FROM_IPT = 2 } p2r_option;
+ *   output block carries it.
+ * P2R_OPT_CHI2_PER_LAYER: `chi2` (when given) receives each update's chi2
+ *   (the reference's Chi2Similarity, commented out at tbb.cpp:631) instead of
+ *   the per-track sum: chi2[L * n + i] for layer L and track (or slot) i, with
+ *   n = nevts*ntrks in p2r_forward and the slot count in p2r_forward_ragged;
+ *   absent layers of an uneven batch get 0.  The buffer holds nlayer * n
+ *   floats. */
+typedef enum p2r_option {
+  P2R_OPT_WRAP_PHI = 1,
+  P2R_OPT_CHARGE_FROM_IPT = 2,
+  P2R_OPT_CHI2_PER_LAYER = 4
+} p2r_option;
 
 /* Context: device binding and workspace.  Replaces the reference's
  * p2r_get_compute_device_id / p2r_get_streams set-up

@@ -91,11 +91,12 @@ class Oracle:
     def run_f32(self, trk, hits, nblk, nlayer, bs, semantics=0, kfact=None, nthreads=0, chi2=False, mask=None,
                 cr_trig=False, options=0):
         """cr_trig: sin/cos correctly rounded (FP64 rounded to float) instead of libm's sinf/cosf.
-        options: 1 = wrap phi after each update, 2 = charge from the sign of 1/pt (see run_block)."""
+        options: 1 = wrap phi after each update, 2 = charge from the sign of 1/pt, 4 = chi2 per layer
+        ([nlayer][nblk*bs], 0 for absent layers) instead of the per-track sum (see run_block)."""
         kfact = self.kfact_tbb() if kfact is None else kfact
         self.lib.orc_set_trig_mode(1 if cr_trig else 0)
         out = np.zeros_like(trk)
-        c2 = np.zeros(nblk * bs, np.float32) if chi2 else None
+        c2 = np.zeros(nblk * bs * (nlayer if options & 4 else 1), np.float32) if chi2 else None
         m = None if mask is None else np.ascontiguousarray(mask, np.uint32)
         rc = self.lib.orc_run_f32(_ptr(trk), _ptr(hits), _ptr(out), nblk, nlayer, bs, semantics, kfact, nthreads,
                                   _ptr(c2), _ptr(m), int(options))

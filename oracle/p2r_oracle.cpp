@@ -52,7 +52,9 @@ inline void store_trk(float* blk, int bs, int it, const Trk<float>& t, int q) {
 // are skipped (chained semantics only).
 // options (bitwise): 1 = wrap phi after each update, phi - 2pi floor((phi + pi) * (1/2pi)) in T
 // (the squash the reference leaves commented out, tbb.cpp:635); 2 = an update that makes 1/pt negative
-// flips 1/pt's sign and the charge (tbb.cpp:527 note), and later layers use the new charge.
+// flips 1/pt's sign and the charge (tbb.cpp:527 note), and later layers use the new charge; 4 = chi2
+// receives each update's chi2 (Chi2Similarity, commented out at tbb.cpp:631) at chi2[l * lstride + it]
+// instead of the per-track sum, 0 for absent layers.
 template <class T>
 T wrap_phi(T phi) {
   const T n = std::floor((phi + T(3.14159265358979323846)) * T(0.15915494309189533577));
@@ -61,7 +63,7 @@ T wrap_phi(T phi) {
 
 template <class T>
 void run_block(const float* tb, const float* hb0, int bs, int nlayer, int semantics, T kf,
-               const uint32_t* mask, Trk<T>* out, int* qout, T* chi2, int options = 0) {
+               const uint32_t* mask, Trk<T>* out, int* qout, T* chi2, int options = 0, long lstride = 0) {
   for (int it = 0; it < bs; ++it) {
     Trk<T> in;
     int q_in;
@@ -70,7 +72,10 @@ void run_block(const float* tb, const float* hb0, int bs, int nlayer, int semant
     Trk<T> cur = in;
     T c2 = T(0.f);
     for (int l = 0; l < nlayer; ++l) {
-      if (mask && !((mask[it] >> l) & 1u)) continue;
+      if (mask && !((mask[it] >> l) & 1u)) {
+        if (chi2 && (options & 4)) chi2[l * lstride + it] = T(0.f);
+        continue;
+      }
       Hit<T> h;
       load_hit(hb0 + (long)l * 9 * bs, bs, it, h);
       const Trk<T>& src = (semantics == 0) ? in : cur;
@@ -84,12 +89,13 @@ void run_block(const float* tb, const float* hb0, int bs, int nlayer, int semant
         prop.par[3] = -prop.par[3];
         q = -q;
       }
+      if (chi2 && (options & 4)) chi2[l * lstride + it] = c2l;
       c2 = c2 + c2l;
       cur = prop;
     }
     out[it] = cur;
     qout[it] = q;
-    if (chi2) chi2[it] = c2;
+    if (chi2 && !(options & 4)) chi2[it] = c2;
   }
 }
 
@@ -183,7 +189,8 @@ void orc_set_trig_mode(int mode) { trig_mode() = mode ? 1 : 0; }
 // FP32 run over nblk Matriplex blocks in the reference layout (the driver loop
 // of tbb.cpp:873-925 for one repetition).  `nblk` = nevts*nb.  chi2 (nullable)
 // receives per-track sum of local chi2 over the processed layers, indexed by
-// block*bs + lane.  mask (nullable) = per-track layer bitmask, same indexing.
+// block*bs + lane (options & 4: each layer's chi2, [nlayer][nblk*bs]).  mask
+// (nullable) = per-track layer bitmask, same indexing.
 int orc_run_f32(const float* trk, const float* hits, float* out, long nblk, int nlayer, int bs, int semantics,
                 float kfact, int nthreads, float* chi2, const uint32_t* mask, int options) {
   if (bs <= 0 || nlayer <= 0 || nblk < 0) return 1;
@@ -195,7 +202,8 @@ int orc_run_f32(const float* trk, const float* hits, float* out, long nblk, int 
     std::vector<Trk<float>> res(bs);
     std::vector<int> q(bs);
     run_block<float>(trk_block(trk, b, bs), hit_block(hits, b * nlayer, bs), bs, nlayer, semantics, kfact,
-                     mask ? mask + b * bs : nullptr, res.data(), q.data(), chi2 ? chi2 + b * bs : nullptr, options);
+                     mask ? mask + b * bs : nullptr, res.data(), q.data(), chi2 ? chi2 + b * bs : nullptr, options,
+                     nblk * bs);
     float* ob = trk_block(out, b, bs);
     for (int it = 0; it < bs; ++it) store_trk(ob, bs, it, res[it], q[it]);
   }
@@ -216,7 +224,7 @@ int orc_run_f64(const float* trk, const float* hits, double* par_out, double* co
     std::vector<int> q(bs);
     std::vector<double> c2(bs);
     run_block<double>(trk_block(trk, b, bs), hit_block(hits, b * nlayer, bs), bs, nlayer, semantics, kfact,
-                      mask ? mask + b * bs : nullptr, res.data(), q.data(), c2.data(), options);
+                      mask ? mask + b * bs : nullptr, res.data(), q.data(), c2.data(), options & 3);
     for (int it = 0; it < bs; ++it) {
       const long t = b * bs + it;
       for (int f = 0; f < 6; ++f) par_out[t * 6 + f] = res[it].par[f];

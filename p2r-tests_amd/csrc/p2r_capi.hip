@@ -29,6 +29,7 @@ struct p2r_graph {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kAllOptions = P2R_OPT_WRAP_PHI | P2R_OPT_CHARGE_FROM_IPT | P2R_OPT_CHI2_PER_LAYER;
 
 __device__ __forceinline__ void load_trk(const float* tb, uint32_t bs, Trk& t) {
 #pragma unroll
@@ -74,8 +75,9 @@ __global__ __launch_bounds__(kThreads, MINW) void fwd_uniform_kernel(const float
   const float* hb = hits + (size_t)blk * nlayer * 9 * bs + lane;
   const size_t hstride = (size_t)9 * bs;
   Trk cur = in;
+  const bool per_layer = opts & P2R_OPT_CHI2_PER_LAYER;
   // a later launch of a split layer loop continues the running sum (same rounding as one launch)
-  float c2 = (CHI2 && acc) ? chi2[g] : 0.f;
+  float c2 = (CHI2 && acc && !per_layer) ? chi2[g] : 0.f;
   Hit hn;
   load_hit(hb + (size_t)l0 * hstride, bs, hn);
   for (int l = l0; l < l1; ++l) {
@@ -101,13 +103,16 @@ __global__ __launch_bounds__(kThreads, MINW) void fwd_uniform_kernel(const float
       opaque(prop);
       opaque(c);
     }
-    if (CHI2) c2 += c;
+    if (CHI2) {
+      if (per_layer) chi2[(size_t)l * ntrk + g] = c;
+      else c2 += c;
+    }
     cur = prop;
   }
   float* ob = out + (size_t)blk * 28 * bs + lane;
   store_trk(ob, bs, cur);
   reinterpret_cast<int*>(ob)[(size_t)27 * bs] = q;
-  if (CHI2) chi2[g] = c2;
+  if (CHI2 && !per_layer) chi2[g] = c2;
 }
 
 
@@ -125,7 +130,10 @@ constexpr int kRagWaves = kRagThreads / 64;
 template <int SEM, bool CHI2, bool EXACT>
 __device__ __forceinline__ void ragged_slot(const float* __restrict__ trk, const float* __restrict__ hits,
                                             float* __restrict__ out, float* __restrict__ chi2, uint32_t slot,
-                                            uint32_t m, uint32_t bs, int nlayer, float kfact, int opts) {
+                                            uint32_t m, uint32_t bs, int nlayer, float kfact, int opts,
+                                            uint32_t nslot) {
+  const bool per_layer = opts & P2R_OPT_CHI2_PER_LAYER;
+  const uint32_t m0 = m;
   const uint32_t blk = slot / bs, lane = slot - blk * bs;
   const float* tb = trk + (size_t)blk * 28 * bs + lane;
   Trk in;
@@ -140,6 +148,7 @@ __device__ __forceinline__ void ragged_slot(const float* __restrict__ trk, const
   Hit hn;
   if (m) load_hit(hb + (size_t)__builtin_ctz(m) * hstride, bs, hn);
   while (m) {
+    const int l = __builtin_ctz(m);
     m &= m - 1u;
     const Hit h = hn;
     if (m) load_hit(hb + (size_t)__builtin_ctz(m) * hstride, bs, hn);
@@ -160,13 +169,21 @@ __device__ __forceinline__ void ragged_slot(const float* __restrict__ trk, const
       opaque(prop);
       opaque(c);
     }
-    if (CHI2) c2 += c;
+    if (CHI2) {
+      if (per_layer) chi2[(size_t)l * nslot + slot] = c;
+      else c2 += c;
+    }
     cur = prop;
   }
   float* ob = out + (size_t)blk * 28 * bs + lane;
   store_trk(ob, bs, cur);
   reinterpret_cast<int*>(ob)[(size_t)27 * bs] = q;
-  if (CHI2) chi2[slot] = c2;
+  if (CHI2) {
+    if (!per_layer) chi2[slot] = c2;
+    else
+      for (int l = 0; l < nlayer; ++l)
+        if (!((m0 >> l) & 1u)) chi2[(size_t)l * nslot + slot] = 0.f;
+  }
 }
 
 template <int SEM, bool CHI2, bool EXACT, int ROUNDS>
@@ -232,7 +249,8 @@ __global__ __launch_bounds__(kRagThreads) void fwd_ragged_kernel(const float* __
       const uint32_t rank = r * kRagThreads + tid;
       if (rank < nvalid) {
         const uint32_t local = order[rank];
-        ragged_slot<SEM, CHI2, EXACT>(trk, hits, out, chi2, base + local, smask[local], bs, nlayer, kfact, opts);
+        ragged_slot<SEM, CHI2, EXACT>(trk, hits, out, chi2, base + local, smask[local], bs, nlayer, kfact, opts,
+                                      nslot);
       }
     }
     __syncthreads();  // order/smask/hist are rewritten by the next tile
@@ -262,6 +280,7 @@ __global__ __launch_bounds__(kThreads) void fwd_ragged_layer_kernel(const float*
   for (int o = 1; o < 64; o <<= 1) wm |= (uint32_t)__shfl_xor((int)wm, o);
   wm = __builtin_amdgcn_readfirstlane(wm);
   if (!active) return;
+  const bool per_layer = opts & P2R_OPT_CHI2_PER_LAYER;
   const uint32_t blk = g / bs, lane = g - blk * bs;
   const float* tb = trk + (size_t)blk * 28 * bs + lane;
   Trk in;
@@ -299,14 +318,22 @@ __global__ __launch_bounds__(kThreads) void fwd_ragged_layer_kernel(const float*
         opaque(prop);
         opaque(c);
       }
-      if (CHI2) c2 += c;
+      if (CHI2) {
+        if (per_layer) chi2[(size_t)l * nslot + g] = c;
+        else c2 += c;
+      }
       cur = prop;
     }
   }
   float* ob = out + (size_t)blk * 28 * bs + lane;
   store_trk(ob, bs, cur);
   reinterpret_cast<int*>(ob)[(size_t)27 * bs] = q;
-  if (CHI2) chi2[g] = c2;
+  if (CHI2) {
+    if (!per_layer) chi2[g] = c2;
+    else
+      for (int l = 0; l < nlayer; ++l)
+        if (!((m >> l) & 1u)) chi2[(size_t)l * nslot + g] = 0.f;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -385,6 +412,7 @@ __global__ __launch_bounds__(64 * kPsWaves) void fwd_ragged_stream_kernel(const 
                                                                            uint32_t nslot, uint32_t bs, int nlayer,
                                                                            float kfact, int opts) {
   constexpr int kW = CHI2 ? kPsWords : kPsWords - 1;  // the chi2 word only when it is returned
+  const bool per_layer = opts & P2R_OPT_CHI2_PER_LAYER;
   __shared__ float st_all[kPsWaves][kW][kPsTile];
   __shared__ uint32_t tbl_all[kPsWaves][65];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -447,7 +475,10 @@ __global__ __launch_bounds__(64 * kPsWaves) void fwd_ragged_stream_kernel(const 
         opaque(prop);
         opaque(cc);
       }
-      if (CHI2) st[kW - 1][slot] += cc;
+      if (CHI2) {
+        if (per_layer) chi2[(size_t)((code >> 7) & 31u) * nslot + base + slot] = cc;
+        else st[kW - 1][slot] += cc;
+      }
       if (SEM == P2R_CHAINED) {
 #pragma unroll
         for (int f = 0; f < 6; ++f) st[f][slot] = prop.par[f];
@@ -471,7 +502,12 @@ __global__ __launch_bounds__(64 * kPsWaves) void fwd_ragged_stream_kernel(const 
 #pragma unroll
       for (int f = 0; f < 28; ++f) ob[(size_t)f * bs] = st[f][slot];
     }
-    if (CHI2) chi2[g] = st[kW - 1][slot];
+    if (CHI2) {
+      if (!per_layer) chi2[g] = st[kW - 1][slot];
+      else
+        for (int l = 0; l < nlayer; ++l)
+          if (!((m[r] >> l) & 1u)) chi2[(size_t)l * nslot + g] = 0.f;
+    }
   }
 }
 
@@ -1050,7 +1086,7 @@ int p2r_forward(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out,
   if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
   if (ntrk == 0) return P2R_OK;
   if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
-  if (p.options & ~(P2R_OPT_WRAP_PHI | P2R_OPT_CHARGE_FROM_IPT)) return c->fail(P2R_ERR_ARG, "bad options %d", p.options);
+  if (p.options & ~kAllOptions) return c->fail(P2R_ERR_ARG, "bad options %d", p.options);
   DeviceGuard g(c->device);
   const UniformLaunch a{static_cast<const float*>(trk_in), static_cast<const float*>(hits), static_cast<float*>(trk_out),
                         chi2, (uint32_t)ntrk, (uint32_t)s->bsize, s->nlayer, p.kfact / p.b_scale, p.options};
@@ -1076,7 +1112,7 @@ int p2r_forward_streamed(p2r_ctx* c, const void* host_trk_in, const void* host_h
   const p2r_params p = params ? *params : p2r_default_params();
   if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
   if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
-  if (p.options & ~(P2R_OPT_WRAP_PHI | P2R_OPT_CHARGE_FROM_IPT)) return c->fail(P2R_ERR_ARG, "bad options %d", p.options);
+  if (p.options & ~kAllOptions) return c->fail(P2R_ERR_ARG, "bad options %d", p.options);
   if (nblk == 0) return P2R_OK;
   DeviceGuard g(c->device);
   const size_t bs = (size_t)s->bsize, tw = 28 * bs * 4, hw = (size_t)s->nlayer * 9 * bs * 4;  // bytes per block
@@ -1132,7 +1168,7 @@ int p2r_forward_ragged(p2r_ctx* c, const void* trk_in, const void* hits, void* t
   const p2r_params p = params ? *params : p2r_default_params();
   if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
   if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
-  if (p.options & ~(P2R_OPT_WRAP_PHI | P2R_OPT_CHARGE_FROM_IPT)) return c->fail(P2R_ERR_ARG, "bad options %d", p.options);
+  if (p.options & ~kAllOptions) return c->fail(P2R_ERR_ARG, "bad options %d", p.options);
   if (nslot == 0) return P2R_OK;
   const int rounds = tile_rounds != P2R_RAGGED_AUTO ? tile_rounds : c->tune.ragged_schedule;
   DeviceGuard g(c->device);

@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._capi import (P2R_CHAINED, P2R_EXACT, P2R_FAST, P2R_GRAD_F32, P2R_GRAD_MIXED, P2R_GRAD_F64, P2R_RAGGED_AUTO, P2R_RAGGED_LAYER_ORDER, P2R_RAGGED_PAIR_STREAM, P2R_OPT_WRAP_PHI, P2R_OPT_CHARGE_FROM_IPT, P2R_ERR_ARG, P2R_ERR_HIP, P2R_ERR_NOMEM, P2R_ERR_SHAPE, P2R_ERR_UNSUPPORTED,
+from ._capi import (P2R_CHAINED, P2R_EXACT, P2R_FAST, P2R_GRAD_F32, P2R_GRAD_MIXED, P2R_GRAD_F64, P2R_RAGGED_AUTO, P2R_RAGGED_LAYER_ORDER, P2R_RAGGED_PAIR_STREAM, P2R_OPT_WRAP_PHI, P2R_OPT_CHARGE_FROM_IPT, P2R_OPT_CHI2_PER_LAYER, P2R_ERR_ARG, P2R_ERR_HIP, P2R_ERR_NOMEM, P2R_ERR_SHAPE, P2R_ERR_UNSUPPORTED,
                     P2R_OK, P2R_REFERENCE, P2RError, lib, p2r_gen_config, p2r_params, p2r_shape, p2r_tune_report,
                     p2r_tuning)
 from .matriplex import Shape, tracks_to_arrays, hits_block_view  # noqa: F401
@@ -39,7 +39,7 @@ GRAD_PRECISION = {"f32": P2R_GRAD_F32, "mixed": P2R_GRAD_MIXED, "f64": P2R_GRAD_
 
 
 def default_params(precision="exact", b_scale=1.0, options=0):
-    """options: bitwise OR of P2R_OPT_WRAP_PHI, P2R_OPT_CHARGE_FROM_IPT (include/p2r.h)."""
+    """options: bitwise OR of P2R_OPT_WRAP_PHI, P2R_OPT_CHARGE_FROM_IPT, P2R_OPT_CHI2_PER_LAYER (include/p2r.h)."""
     p = lib().p2r_default_params()
     prec = PRECISION[precision] if isinstance(precision, str) else int(precision)
     return p2r_params(p.kfact, float(b_scale), prec, int(options))

@@ -13,7 +13,7 @@ P2R_REFERENCE, P2R_CHAINED = 0, 1
 P2R_EXACT, P2R_FAST = 0, 1
 P2R_GRAD_F32, P2R_GRAD_MIXED, P2R_GRAD_F64 = 0, 1, 2
 P2R_RAGGED_AUTO, P2R_RAGGED_LAYER_ORDER, P2R_RAGGED_PAIR_STREAM = 0, -1, -2
-P2R_OPT_WRAP_PHI, P2R_OPT_CHARGE_FROM_IPT = 1, 2
+P2R_OPT_WRAP_PHI, P2R_OPT_CHARGE_FROM_IPT, P2R_OPT_CHI2_PER_LAYER = 1, 2, 4
 
 
 class p2r_shape(C.Structure):

@@ -407,3 +407,49 @@ def test_exact_full_headline_shape_matches_oracle(orc, ctx, semantics):
         check_exact(orc, ctx, trk, hits, shape, semantics, "full")
     finally:
         ctx.set_tuning(fwd_waves=0)
+
+
+# ---- chi2 per layer (P2R_OPT_CHI2_PER_LAYER) ---------------------------------
+
+@pytest.mark.parametrize("per", [0, 7])
+@pytest.mark.parametrize("semantics", ["reference", "chained"])
+def test_exact_chi2_per_layer_matches_oracle(orc, ctx, semantics, per):
+    """Every update's chi2, [layer][track], bit-identical to the oracle; the state output is unchanged by the
+    option; also with the layer loop split over launches."""
+    import p2r_amd
+    shape = Shape(2, 640, 20, 32)
+    trk, hits = p2r_amd.gen_uniform(shape)
+    sem = 0 if semantics == "reference" else 1
+    n = shape.ntrk_total
+    dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
+    do = torch.empty_like(dt)
+    dc = torch.full((20 * n,), float("nan"), device="cuda")
+    ctx.set_tuning(fwd_layers_per_launch=per)
+    try:
+        ctx.forward(dt, dh, do, shape, semantics, params=p2r_amd.default_params("exact", options=4), chi2=dc)
+        torch.cuda.synchronize()
+    finally:
+        ctx.set_tuning(fwd_layers_per_launch=0)
+    cpu, c2 = orc.run_f32(trk, hits, shape.nblk, 20, 32, semantics=sem, chi2=True, cr_trig=True, options=4)
+    assert bit_mismatches(do.cpu().numpy(), cpu) == 0
+    assert bit_mismatches(dc.cpu().numpy(), c2) == 0
+
+
+@pytest.mark.parametrize("rounds", [-1, -2, 1])
+@pytest.mark.parametrize("semantics", ["chained", "reference"])
+def test_exact_ragged_chi2_per_layer_matches_oracle(orc, ctx, semantics, rounds):
+    """Uneven batch with chi2 per layer: present layers bit-identical to the oracle, absent layers 0."""
+    import p2r_amd
+    counts, trk, hits, mask, offs = _ragged_inputs(nevts=10)
+    nblk, nslot = int(offs[-1]), mask.size
+    sem = 0 if semantics == "reference" else 1
+    dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
+    dm = torch.from_numpy(mask.view(np.int32)).cuda()
+    do = torch.empty_like(dt)
+    dc = torch.full((20 * nslot,), float("nan"), device="cuda")
+    ctx.forward_ragged(dt, dh, do, offs, dm, 20, 32, semantics, params=p2r_amd.default_params("exact", options=4),
+                       chi2=dc, tile_rounds=rounds)
+    torch.cuda.synchronize()
+    cpu, c2 = orc.run_f32(trk, hits, nblk, 20, 32, semantics=sem, chi2=True, mask=mask, cr_trig=True, options=4)
+    assert bit_mismatches(do.cpu().numpy(), cpu) == 0
+    assert bit_mismatches(dc.cpu().numpy(), c2) == 0

@@ -254,3 +254,43 @@ def test_option_charge_from_ipt(orc):
     r0 = orc.run_f32(trk, hits, shape.nblk, 20, 32, semantics=0).reshape(-1, 28, 32)
     negr = r0[:, 3] < 0
     assert np.array_equal(r2[:, 27].view(np.int32)[negr], -r0[:, 27].view(np.int32)[negr])
+
+
+def test_option_chi2_per_layer(orc):
+    """options=4: chi2 per (layer, track), layer-major.  Pinned three ways: (1) summed over layers in layer
+    order (FP32, sequential) it equals the per-track sum bit for bit; (2) in reference semantics layer L
+    restarts from the input, so its chi2 equals a one-layer run on hit L alone; (3) with a mask common to
+    all tracks, the masked chained run's chi2 at the present layers equals the unmasked run over just those
+    layers' hits, and absent layers read 0."""
+    import p2r_amd
+    from p2r_amd import Shape
+    shape = Shape(1, 128, 20, 32)
+    trk, hits = p2r_amd.gen_uniform(shape)
+    n, nl = shape.ntrk_total, 20
+    for sem in (0, 1):
+        _, tot = orc.run_f32(trk, hits, shape.nblk, nl, 32, semantics=sem, chi2=True)
+        _, per = orc.run_f32(trk, hits, shape.nblk, nl, 32, semantics=sem, chi2=True, options=4)
+        per = per.reshape(nl, n)
+        acc = np.zeros(n, np.float32)
+        for l in range(nl):
+            acc = acc + per[l]
+        assert np.array_equal(acc.view(np.uint32), tot.view(np.uint32))
+    H = hits.reshape(shape.nblk, nl, 9, 32)
+    _, per0 = orc.run_f32(trk, hits, shape.nblk, nl, 32, semantics=0, chi2=True, options=4)
+    per0 = per0.reshape(nl, n)
+    for L in (0, 7, 19):
+        hL = np.ascontiguousarray(H[:, L:L + 1]).ravel()
+        _, one = orc.run_f32(trk, hL, shape.nblk, 1, 32, semantics=0, chi2=True)
+        assert np.array_equal(one.view(np.uint32), per0[L].view(np.uint32))
+    present = [1, 2, 5, 11, 12, 18]
+    bits = sum(1 << l for l in present)
+    mask = np.full(n, bits, np.uint32)
+    _, pm = orc.run_f32(trk, hits, shape.nblk, nl, 32, semantics=1, chi2=True, mask=mask, options=4)
+    pm = pm.reshape(nl, n)
+    hc = np.ascontiguousarray(H[:, present]).ravel()
+    _, pc = orc.run_f32(trk, hc, shape.nblk, len(present), 32, semantics=1, chi2=True, options=4)
+    pc = pc.reshape(len(present), n)
+    for j, l in enumerate(present):
+        assert np.array_equal(pm[l].view(np.uint32), pc[j].view(np.uint32))
+    absent = [l for l in range(nl) if l not in present]
+    assert np.all(pm[absent] == 0)
