@@ -49,6 +49,14 @@ __device__ __forceinline__ void load_hit(const float* hb, uint32_t bs, Hit& h) {
 #pragma unroll
   for (int f = 0; f < 6; ++f) h.cov[f] = __builtin_nontemporal_load(hb + (size_t)(3 + f) * bs);
 }
+// Hit loads through the caches with the default policy: for schedules whose hit lines are shared by
+// consecutive steps of one wavefront.
+__device__ __forceinline__ void load_hit_cached(const float* hb, uint32_t bs, Hit& h) {
+#pragma unroll
+  for (int f = 0; f < 3; ++f) h.pos[f] = hb[(size_t)f * bs];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) h.cov[f] = hb[(size_t)(3 + f) * bs];
+}
 
 // Whole batch, all layers per lane.  SEM: P2R_REFERENCE / P2R_CHAINED;
 // EXACT selects the per-lane math instance (p2r_lane.hpp).
@@ -448,14 +456,14 @@ __global__ __launch_bounds__(64 * kPsWaves) void fwd_ragged_stream_kernel(const 
   uint32_t c = 0, cn = 0;
   uint32_t code_n = ps_step(c, m, rb, ln, lane, tbl, cn);
   Hit hn;
-  if (code_n != kPsNone) load_hit(hit_ptr(code_n), bs, hn);
+  if (code_n != kPsNone) load_hit_cached(hit_ptr(code_n), bs, hn);
   while (c < cend) {
     const uint32_t code = code_n;
     const Hit h = hn;
     c = cn;
     if (c < cend) {  // next step's pairs and hit loads, issued before this step's arithmetic
       code_n = ps_step(c, m, rb, ln, lane, tbl, cn);
-      if (code_n != kPsNone) load_hit(hit_ptr(code_n), bs, hn);
+      if (code_n != kPsNone) load_hit_cached(hit_ptr(code_n), bs, hn);
     }
     if (code != kPsNone) {
       const uint32_t slot = code & (kPsTile - 1);
