@@ -58,7 +58,7 @@ def main():
     ap.add_argument("--alignment-tracks", type=int, default=1000000)
     ap.add_argument("--no-parity", action="store_true", help="skip the parity spot check")
     ap.add_argument("--no-autotune", action="store_true", help="time the library's default launch choices")
-    ap.add_argument("--fwd-waves", type=int, default=None, choices=[0, 3, 4],
+    ap.add_argument("--fwd-waves", type=int, default=None, choices=[0, 1, 3, 4],
                     help="set the forward kernels' register budget (waves per SIMD) instead of autotuning")
     ap.add_argument("--no-transfers", action="store_true", help="skip the host-to-host (PCIe-inclusive) timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -394,7 +394,8 @@ def ragged_bench(ctx, args, uniform_chained, headline):
                 g = torch.cuda.CUDAGraph()
                 with torch.cuda.graph(g, stream=stream):
                     ctx.forward_ragged(dt, dh, do, offs, dm, 20, bs, "chained", precision=prec, stream=stream)
-                g.replay()
+                for _ in range(5):  # untimed replays: clocks and caches settle after the autotuner's candidates
+                    g.replay()
                 stream.synchronize()
                 rounds = []
                 for _ in range(3):  # median of 3 rounds of 10 replays

@@ -251,9 +251,10 @@ typedef struct p2r_tuning {
                                     P2R_RAGGED_LAYER_ORDER, 1, 2, 4 */
   int32_t ragged_threads;        /* workgroup size of the layer-ordered schedule: 64, 128 or 256 (256) */
   int32_t checkpoint_stride;     /* what checkpoint_stride = 0 means in p2r_forward_adjoint: 1, 2, 4, 8 (4) */
-  int32_t fwd_waves;             /* p2r_forward kernels compiled for at least this many waves per SIMD:
-                                    0 = the compiler's register allocation (default), 3 or 4 (fewer
-                                    registers, possibly spills) */
+  int32_t fwd_waves;             /* register budget of the p2r_forward kernels, in waves per SIMD:
+                                    0 = the library's choice (default: 3 for P2R_EXACT, the
+                                    compiler's allocation for P2R_FAST), 1 = the compiler's
+                                    allocation, 3 or 4 (fewer registers, possibly spills) */
 } p2r_tuning;
 int p2r_get_tuning(const p2r_ctx* ctx, p2r_tuning* out);
 int p2r_set_tuning(p2r_ctx* ctx, const p2r_tuning* in); /* P2R_ERR_ARG on a value outside the lists above */
@@ -275,7 +276,7 @@ typedef struct p2r_tune_report {
  * synchronize the stream and create events: not for use under graph capture.
  * Outputs are overwritten with valid results of the last candidate run.
  *   forward: fwd_threads {64, 128, 256} x fwd_layers_per_launch {all,
- *            ceil(nlayer/2), ceil(nlayer/4)}, then fwd_waves {0, 3, 4} at the
+ *            ceil(nlayer/2), ceil(nlayer/4)}, then fwd_waves {1, 3, 4} at the
  *            best of those;
  *   ragged:  ragged_schedule {layer order x ragged_threads {64, 128, 256},
  *            pair stream, sorted tiles of 1, 2, 4 rounds};

@@ -13,7 +13,10 @@
 //             [--NWARMUP N] [--semantics reference|chained]
 //             [--input reference|baseline] [--transfers none|h2d|d2h|both]
 //             [--include-transfers (= both)] [--num-streams N] [--threads 64|128|256]
-//             [--device D]
+//             [--graph] [--device D]
+//
+// --graph captures the kernel-only repetition once (p2r_graph_forward) and
+// replays the graph NITER times; with transfers it is ignored.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -51,6 +54,7 @@ struct Opts {
   int ntrks = 9600, nevts = 100, nlayer = 20, bsize = 32, niter = 5, nwarmup = 2, device = 0, nstreams = 1;
   int semantics = P2R_REFERENCE;
   int threads = 0;  // workgroup size of the forward kernel (0 = library default)
+  bool graph = false;  // replay one captured launch (kernel-only timing)
   bool reference_input = true, h2d = false, d2h = false;
 };
 
@@ -85,6 +89,7 @@ Opts parse(int argc, char** argv) {
       }
     } else if (a == "--num-streams") o.nstreams = std::atoi(next());
     else if (a == "--threads") o.threads = std::atoi(next());
+    else if (a == "--graph") o.graph = true;
     else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       std::exit(1);
@@ -223,7 +228,17 @@ int main(int argc, char** argv) {
   // Timed region as the reference's MEASURE_H2D_TRANSFER / MEASURE_D2H_TRANSFER (hip.cu:17-22,958-1006):
   // both: the streamed form, chunks of blocks over num_streams streams; one of them: that copy on the
   // stream around the kernel (the other happens outside the timed region); none: the kernel only.
+  p2r_graph* graph = nullptr;
+  if (o.graph && !o.h2d && !o.d2h)
+    CHECK_P2R(ctx, p2r_graph_forward(ctx, d_trk, d_hit, d_out, &shape, o.semantics, &params, nullptr, &graph));
   auto one = [&]() {
+    if (graph) {
+      if (p2r_graph_launch(graph, st) != P2R_OK) {
+        std::fprintf(stderr, "p2r_graph_launch failed\n");
+        std::exit(1);
+      }
+      return;
+    }
     if (o.h2d && o.d2h) {
       CHECK_P2R(ctx, p2r_forward_streamed(ctx, h_trk, h_hit, h_out, d_trk, d_hit, d_out, &shape, o.semantics, &params,
                                           o.nstreams));
@@ -251,6 +266,7 @@ int main(int argc, char** argv) {
   std::printf("track-layer updates/s=%e\n", (double)ntot * o.nlayer / wall);
   print_summary(std::vector<float>(h_out, h_out + trk_floats), o.nevts, o.ntrks, o.bsize, hit_ref);
 
+  if (graph) p2r_graph_destroy(graph);
   p2r_destroy(ctx);
   CHECK_HIP(hipFree(d_trk));
   CHECK_HIP(hipFree(d_hit));

@@ -798,6 +798,9 @@ void launch_uniform_w(dim3 grid, dim3 block, hipStream_t st, const UniformLaunch
 template <int SEM, bool CHI2, bool EXACT>
 void launch_uniform_e(int waves, dim3 grid, dim3 block, hipStream_t st, const UniformLaunch& a, const float* in,
                       int l0, int l1, bool acc) {
+  // 0 = the library's choice: the 3-wave budget for the exact instance (faster on MI355X in both
+  // semantics, measured), the compiler's allocation for the fast one
+  if (waves == 0) waves = EXACT ? 3 : 1;
   if (waves == 3) launch_uniform_w<SEM, CHI2, EXACT, 3>(grid, block, st, a, in, l0, l1, acc);
   else if (waves == 4) launch_uniform_w<SEM, CHI2, EXACT, 4>(grid, block, st, a, in, l0, l1, acc);
   else launch_uniform_w<SEM, CHI2, EXACT, 1>(grid, block, st, a, in, l0, l1, acc);

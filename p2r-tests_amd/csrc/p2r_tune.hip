@@ -130,8 +130,8 @@ int p2r_set_tuning(p2r_ctx* c, const p2r_tuning* in) {
   if (!schedule_ok(in->ragged_schedule))
     return c->fail(P2R_ERR_ARG, "ragged_schedule must be P2R_RAGGED_LAYER_ORDER, P2R_RAGGED_PAIR_STREAM, 1, 2 or 4");
   if (!stride_ok(in->checkpoint_stride)) return c->fail(P2R_ERR_ARG, "checkpoint_stride must be 1, 2, 4 or 8");
-  if (in->fwd_waves != 0 && in->fwd_waves != 3 && in->fwd_waves != 4)
-    return c->fail(P2R_ERR_ARG, "fwd_waves must be 0, 3 or 4");
+  if (in->fwd_waves != 0 && in->fwd_waves != 1 && in->fwd_waves != 3 && in->fwd_waves != 4)
+    return c->fail(P2R_ERR_ARG, "fwd_waves must be 0, 1, 3 or 4");
   c->tune = *in;
   return P2R_OK;
 }
@@ -151,6 +151,7 @@ int p2r_autotune_forward(p2r_ctx* c, const void* trk_in, const void* hits, void*
       p2r_tuning t = keep;
       t.fwd_threads = threads;
       t.fwd_layers_per_launch = per[pi];
+      t.fwd_waves = 0;
       cand.push_back(t);
     }
   }
@@ -163,17 +164,16 @@ int p2r_autotune_forward(p2r_ctx* c, const void* trk_in, const void* hits, void*
   }
   // register budget at the best workgroup size / layer split
   const size_t b0 = std::min_element(ms.begin(), ms.end()) - ms.begin();
-  for (int waves : {3, 4}) {
+  for (int waves : {1, 3, 4}) {
     p2r_tuning t = cand[b0];
     t.fwd_waves = waves;
     cand.push_back(t);
   }
-  float wms[2] = {};
-  rc = time_candidates(c, st, reps, 2, [&](int i) { c->tune = cand[cand.size() - 2 + i]; }, run, wms);
+  float wms[3] = {};
+  rc = time_candidates(c, st, reps, 3, [&](int i) { c->tune = cand[cand.size() - 3 + i]; }, run, wms);
   c->tune = keep;
   if (rc != P2R_OK) return rc;
-  ms.push_back(wms[0]);
-  ms.push_back(wms[1]);
+  for (float w : wms) ms.push_back(w);
   Collect col;
   for (size_t i = 0; i < cand.size(); ++i) {
     p2r_tune_entry e = entry_of(cand[i], s->bsize, ms[i], 0);
