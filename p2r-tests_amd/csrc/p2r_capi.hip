@@ -568,15 +568,64 @@ template <class R> __device__ __forceinline__ void load_hit_as(const float* hb, 
   convert(hf, h);
 }
 
+// Forward pass of the adjoint: the chained fit in the state precision RS, storing the state every CS
+// layers (checkpoint 0 is the input track, not stored) and each track's total chi2 (for the reverse
+// kernel), and writing the fitted track and chi2 as p2r_forward does.  Its own kernel so it runs at the
+// forward's occupancy (MINW waves per SIMD), not at the reverse sweep's register budget.
+template <class RS, bool EXACT, int CS, int MINW>
+__global__ __launch_bounds__(kAdjThreads, MINW) void adjoint_fwd_kernel(
+    const float* __restrict__ trk, const float* __restrict__ hits, float* __restrict__ out, float* __restrict__ chi2,
+    RS* __restrict__ ckpt, RS* __restrict__ c2_total, uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
+  using MS = MathOf<RS, EXACT>;
+  const uint32_t g = blockIdx.x * kAdjThreads + threadIdx.x;
+  if (g >= ntrk) return;
+  const uint32_t blk = g / bs, lane = g - blk * bs;
+  const float* tb = trk + (size_t)blk * 28 * bs + lane;
+  const float* hb = hits + (size_t)blk * nlayer * 9 * bs + lane;
+  const size_t hstride = (size_t)9 * bs;
+  const int q = reinterpret_cast<const int*>(tb)[(size_t)27 * bs];
+  const float k = (float)q * kf;
+  Trk in_f;
+  load_trk(tb, bs, in_f);
+  TrkT<RS> cur;
+  convert(in_f, cur);
+  RS c2 = 0;
+  HitT<RS> hn;
+  load_hit_as(hb, bs, hn);
+  for (int l = 0; l < nlayer; ++l) {
+    if (l % CS == 0 && l > 0) {
+      RS* cp = ckpt + ((size_t)(l / CS - 1) * 27) * ntrk + g;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) cp[(size_t)f * ntrk] = cur.par[f];
+#pragma unroll
+      for (int f = 0; f < 21; ++f) cp[(size_t)(6 + f) * ntrk] = cur.cov[f];
+    }
+    const HitT<RS> h = hn;
+    if (l + 1 < nlayer) load_hit_as(hb + (l + 1) * hstride, bs, hn);
+    TrkT<RS> nxt;
+    c2 += MS::fwd(cur, (RS)k, h, nxt);
+    cur = nxt;
+  }
+  float* ob = out + (size_t)blk * 28 * bs + lane;
+  Trk of;
+  convert(cur, of);
+  store_trk(ob, bs, of);
+  reinterpret_cast<int*>(ob)[(size_t)27 * bs] = q;
+  if (chi2) chi2[g] = (float)c2;
+  c2_total[g] = c2;
+}
+
+// Reverse sweep of the adjoint, after adjoint_fwd_kernel: per segment, reload the checkpoint, recompute
+// the segment's layer inputs (registers), then per layer recompute its values and back-propagate.
 template <class RS, class RL, bool EXACT, int CS>
 __global__ __launch_bounds__(kAdjThreads) void adjoint_kernel(
-    const float* __restrict__ trk, const float* __restrict__ hits, float* __restrict__ out, float* __restrict__ chi2,
+    const float* __restrict__ trk, const float* __restrict__ hits, const RS* __restrict__ c2_total,
     float* __restrict__ gpar, float* __restrict__ ghit, float* __restrict__ gk, double* __restrict__ partials,
-    RS* __restrict__ ckpt, uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
+    const RS* __restrict__ ckpt, uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
   using MS = MathOf<RS, EXACT>;
-  // The forward pass and the segment recomputation run in the requested arithmetic (outputs and chi2 as
-  // p2r_forward's); the reverse sweep always in the fused instance: gradients have no reference rounding
-  // to reproduce, and they are checked against FP64 to a tolerance.
+  // The segment recomputation runs in the requested arithmetic (the layer inputs of the forward pass);
+  // the reverse sweep always in the fused instance: gradients have no reference rounding to reproduce,
+  // and they are checked against FP64 to a tolerance.
   using ML = MathOf<RL, false>;
   constexpr int kWaves = kAdjThreads / 64;
   extern __shared__ double lds_global[];  // [kWaves][4*nlayer + 1] when partials != nullptr
@@ -589,35 +638,8 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_kernel(
   const size_t hstride = (size_t)9 * bs;
   const int q = reinterpret_cast<const int*>(tb)[(size_t)27 * bs];
   const float k = (float)q * kf;
-  Trk in_f;
-  load_trk(tb, bs, in_f);
-  TrkT<RS> cur;
-  convert(in_f, cur);
-  // ---- forward pass with checkpoints ----
-  RS c2 = 0;
   const int nseg = (nlayer + CS - 1) / CS;
-  for (int l = 0; l < nlayer; ++l) {
-    if (l % CS == 0 && l > 0) {
-      RS* cp = ckpt + ((size_t)(l / CS - 1) * 27) * ntrk + gg;
-#pragma unroll
-      for (int f = 0; f < 6; ++f) cp[(size_t)f * ntrk] = cur.par[f];
-#pragma unroll
-      for (int f = 0; f < 21; ++f) cp[(size_t)(6 + f) * ntrk] = cur.cov[f];
-    }
-    HitT<RS> h;
-    load_hit_as(hb + l * hstride, bs, h);
-    TrkT<RS> nxt;
-    c2 += MS::fwd(cur, (RS)k, h, nxt);
-    cur = nxt;
-  }
-  if (active) {
-    float* ob = out + (size_t)blk * 28 * bs + lane;
-    Trk of;
-    convert(cur, of);
-    store_trk(ob, bs, of);
-    reinterpret_cast<int*>(ob)[(size_t)27 * bs] = q;
-    if (chi2) chi2[g] = (float)c2;
-  }
+  const RS c2 = c2_total[gg];
   // ---- reverse sweep ----
   // a track whose fit diverged (non-finite total chi2) adds nothing to the global sums
   const bool in_global = active && isfinite((double)c2);
@@ -632,6 +654,8 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_kernel(
   for (int s = nseg - 1; s >= 0; --s) {
     TrkT<RS> st[CS];
     if (s == 0) {
+      Trk in_f;
+      load_trk(tb, bs, in_f);
       convert(in_f, st[0]);
     } else {
       const RS* cp = ckpt + ((size_t)(s - 1) * 27) * ntrk + gg;
@@ -881,24 +905,29 @@ int launch_ragged_s(bool chi2_on, bool exact, int rounds, p2r_ctx* c, hipStream_
 
 template <class RS, class RL, bool EXACT, int CS>
 void launch_adjoint_t(unsigned grid, size_t lds, hipStream_t st, const float* T, const float* H, float* O, float* chi2,
-                      float* gpar, float* ghit, float* gk, double* partials, void* ckpt, uint32_t ntrk, uint32_t bs,
-                      int nlayer, float kf) {
-  adjoint_kernel<RS, RL, EXACT, CS><<<grid, kAdjThreads, lds, st>>>(T, H, O, chi2, gpar, ghit, gk, partials,
-                                                                    static_cast<RS*>(ckpt), ntrk, bs, nlayer, kf);
+                      float* gpar, float* ghit, float* gk, double* partials, void* ckpt, void* c2_total, uint32_t ntrk,
+                      uint32_t bs, int nlayer, float kf) {
+  // the forward pass at the forward kernels' register budget (3 waves per SIMD in FP32)
+  constexpr int kFwdWaves = sizeof(RS) == sizeof(float) ? 3 : 1;
+  adjoint_fwd_kernel<RS, EXACT, CS, kFwdWaves><<<grid, kAdjThreads, 0, st>>>(
+      T, H, O, chi2, static_cast<RS*>(ckpt), static_cast<RS*>(c2_total), ntrk, bs, nlayer, kf);
+  adjoint_kernel<RS, RL, EXACT, CS><<<grid, kAdjThreads, lds, st>>>(T, H, static_cast<const RS*>(c2_total), gpar,
+                                                                    ghit, gk, partials, static_cast<const RS*>(ckpt),
+                                                                    ntrk, bs, nlayer, kf);
 }
 template <bool EXACT, int CS>
 void launch_adjoint_p(int gp, unsigned grid, size_t lds, hipStream_t st, const float* T, const float* H, float* O,
-                      float* chi2, float* gpar, float* ghit, float* gk, double* partials, void* ckpt, uint32_t ntrk,
-                      uint32_t bs, int nlayer, float kf) {
+                      float* chi2, float* gpar, float* ghit, float* gk, double* partials, void* ckpt, void* c2_total,
+                      uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
   if (gp == P2R_GRAD_F32)
-    launch_adjoint_t<float, float, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt, ntrk, bs,
-                                              nlayer, kf);
+    launch_adjoint_t<float, float, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt, c2_total,
+                                              ntrk, bs, nlayer, kf);
   else if (gp == P2R_GRAD_MIXED)
-    launch_adjoint_t<double, float, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt, ntrk,
-                                               bs, nlayer, kf);
+    launch_adjoint_t<double, float, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt,
+                                               c2_total, ntrk, bs, nlayer, kf);
   else
-    launch_adjoint_t<double, double, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt, ntrk,
-                                                bs, nlayer, kf);
+    launch_adjoint_t<double, double, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt,
+                                                c2_total, ntrk, bs, nlayer, kf);
 }
 
 inline size_t ck_bytes_aligned(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -908,7 +937,7 @@ struct AdjointPlan {
   uint32_t n = 0, bs = 0;
   int nlayer = 0, nglob = 0, cs = 0, grad_precision = 0;
   unsigned grid = 0;
-  size_t ck_bytes = 0, ws_bytes = 0, lds = 0;
+  size_t ck_bytes = 0, c2_bytes = 0, ws_bytes = 0, lds = 0;  // workspace: checkpoints | chi2 totals | partials
   p2r_params p{};
 };
 
@@ -944,8 +973,9 @@ int adjoint_plan(p2r_ctx* c, const void* trk_in, const void* hits, const void* t
   const size_t sz = grad_precision == P2R_GRAD_F32 ? sizeof(float) : sizeof(double);
   const size_t nck = (size_t)((pl.nlayer + cs - 1) / cs - 1);
   pl.ck_bytes = std::max<size_t>(nck, 1) * 27 * ntrk * sz;
+  pl.c2_bytes = ntrk * sz;
   const size_t part_bytes = global ? (size_t)pl.grid * pl.nglob * sizeof(double) : 0;
-  pl.ws_bytes = ck_bytes_aligned(pl.ck_bytes) + part_bytes + 256;
+  pl.ws_bytes = ck_bytes_aligned(pl.ck_bytes) + ck_bytes_aligned(pl.c2_bytes) + part_bytes + 256;
   pl.lds = global ? (size_t)(kAdjThreads / 64) * pl.nglob * sizeof(double) : 0;
   return P2R_OK;
 }
@@ -955,7 +985,9 @@ int adjoint_plan(p2r_ctx* c, const void* trk_in, const void* hits, const void* t
 int adjoint_enqueue(p2r_ctx* c, const AdjointPlan& pl, const void* trk_in, const void* hits, void* trk_out,
                     float* chi2, float* grad_par, float* grad_hits, float* grad_kfact, double* grad_global, char* ws,
                     hipStream_t st) {
-  double* partials = grad_global ? reinterpret_cast<double*>(ws + ck_bytes_aligned(pl.ck_bytes)) : nullptr;
+  char* c2_total = ws + ck_bytes_aligned(pl.ck_bytes);
+  double* partials =
+      grad_global ? reinterpret_cast<double*>(c2_total + ck_bytes_aligned(pl.c2_bytes)) : nullptr;
   const float kf = pl.p.kfact / pl.p.b_scale;
   auto T = static_cast<const float*>(trk_in);
   auto H = static_cast<const float*>(hits);
@@ -964,7 +996,7 @@ int adjoint_enqueue(p2r_ctx* c, const AdjointPlan& pl, const void* trk_in, const
   const bool exact = pl.p.precision == P2R_EXACT;
 #define P2R_ADJ(E, CS)                                                                                         \
   launch_adjoint_p<E, CS>(gp, pl.grid, pl.lds, st, T, H, O, chi2, grad_par, grad_hits, grad_kfact, partials, ws, \
-                          pl.n, pl.bs, pl.nlayer, kf)
+                          c2_total, pl.n, pl.bs, pl.nlayer, kf)
   if (exact) {
     if (cs == 1) P2R_ADJ(true, 1); else if (cs == 2) P2R_ADJ(true, 2); else if (cs == 4) P2R_ADJ(true, 4);
     else P2R_ADJ(true, 8);
