@@ -1,0 +1,111 @@
+"""GPU edge cases of the C-ABI: layer counts other than 20 (1, 2, 3, 40), the largest Matriplex width
+(bsize 1024), adjoints whose last checkpoint segment is shorter than the stride, and empty batches
+through every entry point.  Exact mode is compared word for word with the oracle."""
+import numpy as np
+import pytest
+
+from p2r_amd import Shape
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def orc():
+    import oracle
+    return oracle.Oracle()
+
+
+@pytest.fixture()
+def ctx():
+    import p2r_amd
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = p2r_amd.Context(0)
+    yield c
+    c.close()
+
+
+def _same(a, b):
+    a = np.asarray(a, np.float32).ravel()
+    b = np.asarray(b, np.float32).ravel()
+    return bool(np.all((a.view(np.uint32) == b.view(np.uint32)) | (np.isnan(a) & np.isnan(b))))
+
+
+def _forward(ctx, trk, hits, shape, sem):
+    dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
+    do = torch.empty_like(dt)
+    dc = torch.empty(shape.ntrk_total, device="cuda")
+    ctx.forward(dt, dh, do, shape, sem, chi2=dc, precision="exact")
+    torch.cuda.synchronize()
+    return do.cpu().numpy(), dc.cpu().numpy()
+
+
+@pytest.mark.parametrize("nlayer", [1, 2, 3, 40])
+@pytest.mark.parametrize("semantics", ["reference", "chained"])
+def test_exact_layer_counts(orc, ctx, semantics, nlayer):
+    import p2r_amd
+    shape = Shape(2, 512, nlayer, 32)
+    cfg = p2r_amd.gen_config(r0=20.0, dr=2.5 if nlayer > 20 else 5.0)
+    trk, hits = p2r_amd.gen_uniform(shape, cfg)
+    gpu, c2g = _forward(ctx, trk, hits, shape, semantics)
+    cpu, c2c = orc.run_f32(trk, hits, shape.nblk, nlayer, 32, semantics=0 if semantics == "reference" else 1,
+                           chi2=True, cr_trig=True)
+    assert _same(gpu, cpu) and _same(c2g, c2c)
+
+
+def test_exact_widest_matriplex(orc, ctx):
+    import p2r_amd
+    shape = Shape(1, 2048, 20, 1024)
+    trk, hits = p2r_amd.gen_uniform(shape)
+    for sem in ("reference", "chained"):
+        gpu, c2g = _forward(ctx, trk, hits, shape, sem)
+        cpu, c2c = orc.run_f32(trk, hits, shape.nblk, 20, 1024, semantics=0 if sem == "reference" else 1,
+                               chi2=True, cr_trig=True)
+        assert _same(gpu, cpu) and _same(c2g, c2c)
+
+
+@pytest.mark.parametrize("nlayer", [1, 3])
+def test_adjoint_short_fits_all_strides(ctx, nlayer):
+    """nlayer below the checkpoint stride: every stride gives the same gradients, and the forward outputs
+    equal p2r_forward's."""
+    import p2r_amd
+    shape = Shape(1, 1024, nlayer, 32)
+    trk, hits = p2r_amd.gen_uniform(shape)
+    dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
+    n = shape.ntrk_total
+    ref_out, ref_c2 = _forward(ctx, trk, hits, shape, "chained")
+    res = []
+    for cs in (1, 2, 4, 8):
+        do = torch.empty_like(dt)
+        c2 = torch.empty(n, device="cuda")
+        gpar = torch.empty(6 * n, device="cuda")
+        ghit = torch.empty(3 * n * nlayer, device="cuda")
+        gk = torch.empty(n, device="cuda")
+        gg = torch.empty(4 * nlayer + 1, device="cuda", dtype=torch.float64)
+        ctx.forward_adjoint(dt, dh, do, shape, grad_precision="f32", checkpoint_stride=cs, chi2=c2, grad_par=gpar,
+                            grad_hits=ghit, grad_kfact=gk, grad_global=gg, precision="exact")
+        torch.cuda.synchronize()
+        assert _same(do.cpu().numpy(), ref_out) and _same(c2.cpu().numpy(), ref_c2), cs
+        res.append([t.cpu().numpy().copy() for t in (gpar, ghit, gk, gg)])
+        assert all(np.isfinite(a).all() for a in res[-1])
+    for r in res[1:]:
+        assert all(np.array_equal(a.view(np.uint8), b.view(np.uint8)) for a, b in zip(r, res[0]))
+
+
+def test_empty_batches_are_no_ops(ctx):
+    """nevts = 0 (and an uneven batch of empty events) through every entry point: status OK, buffers
+    untouched."""
+    t = torch.full((28 * 32,), 7.0, device="cuda")
+    h = torch.zeros(9 * 32 * 20, device="cuda")
+    before = t.clone()
+    ctx.forward(t, h, t, Shape(0, 32, 20, 32), "reference")
+    ctx.forward_adjoint(t, h, t, Shape(0, 32, 20, 32))
+    m = torch.zeros(32, dtype=torch.int32, device="cuda")
+    for rounds in (-1, -2, 1):
+        ctx.forward_ragged(t, h, t, np.array([0, 0, 0], np.int64), m, 20, 32, "chained", tile_rounds=rounds)
+    g = ctx.graph_forward_adjoint(t, h, t, Shape(0, 32, 20, 32))
+    g.launch()
+    g.close()
+    torch.cuda.synchronize()
+    assert torch.equal(t, before)
