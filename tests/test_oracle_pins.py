@@ -294,3 +294,52 @@ def test_option_chi2_per_layer(orc):
         assert np.array_equal(pm[l].view(np.uint32), pc[j].view(np.uint32))
     absent = [l for l in range(nl) if l not in present]
     assert np.all(pm[absent] == 0)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_transport_preserves_positive_definiteness(orc, seed):
+    """J P J^T of a positive-definite P stays positive definite (J is non-singular: unit rows 3, 5 and
+    J22 = 1); FP64 lane, random PD covariances around the C0 track, several layers."""
+    rng = np.random.default_rng(100 + seed)
+    d = json.load(open(os.path.join(GOLDEN, "input_track_c0.json")))
+    par = np.array(d["track"]["par"])
+    for layer in (0, 9, 19):
+        P = _random_pd(rng, 6, 1e-4)
+        hit = np.array(d["hits"][21 - layer]["pos"])
+        _, c, J = orc.propagate_lane_f64(par, _sym_pack(P), 1, hit, float(orc.kfact_tbb()))
+        assert abs(np.linalg.det(J)) > 0
+        assert np.linalg.eigvalsh(_sym_full(c)).min() > 0
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_update_does_not_increase_variances(orc, seed):
+    """P' = P - K H P with K = P H^T S^-1 removes a positive semi-definite term: for PD P and PD hit
+    covariance no diagonal variance grows, and P' stays PD."""
+    rng = np.random.default_rng(200 + seed)
+    P = _random_pd(rng, 6, 1e-4)
+    Vf = _random_pd(rng, 3, 1e-5)
+    V = np.array([Vf[0, 0], Vf[1, 0], Vf[1, 1], Vf[2, 0], Vf[2, 1], Vf[2, 2]])
+    par = np.array([-30.0, -20.0, 10.0, 0.3, -2.5, 0.9])
+    hit = np.array([-31.0, -19.5, 10.3])
+    _, c, chi2 = orc.kalman_lane_f64(par, _sym_pack(P), hit, V)
+    Pn = _sym_full(c)
+    assert np.all(np.diag(Pn) <= np.diag(P) * (1 + 1e-12))
+    assert np.linalg.eigvalsh(Pn).min() > 0 and chi2 >= 0
+
+
+def test_masked_run_is_dense_run_on_compacted_hits(orc):
+    """Chained semantics with a layer mask common to a block equals the unmasked run over just the present
+    layers' hits, state and charge bit for bit (absent layers: no propagation, no update)."""
+    import p2r_amd
+    from p2r_amd import Shape
+    shape = Shape(1, 128, 20, 32)
+    trk, hits = p2r_amd.gen_uniform(shape)
+    H = hits.reshape(shape.nblk, 20, 9, 32)
+    for present in ([0, 3, 4, 9, 15], [19], list(range(0, 20, 2))):
+        mask = np.full(shape.ntrk_total, sum(1 << l for l in present), np.uint32)
+        masked = orc.run_f32(trk, hits, shape.nblk, 20, 32, semantics=1, mask=mask)
+        dense = orc.run_f32(trk, np.ascontiguousarray(H[:, present]).ravel(), shape.nblk, len(present), 32,
+                            semantics=1)
+        assert np.array_equal(_bits(masked), _bits(dense)), present
+    none = orc.run_f32(trk, hits, shape.nblk, 20, 32, semantics=1, mask=np.zeros(shape.ntrk_total, np.uint32))
+    assert np.array_equal(_bits(none), _bits(trk))  # no present layer: copied through
