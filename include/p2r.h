@@ -96,7 +96,10 @@ typedef enum p2r_option {
 
 /* Context: device binding and workspace.  Replaces the reference's
  * p2r_get_compute_device_id / p2r_get_streams set-up
- * (propagate-tor-test_hip_local.cu:112-136). */
+ * (propagate-tor-test_hip_local.cu:112-136).  With P2R_ROCTX=1 in the
+ * environment at creation, every entry point called on the context is
+ * wrapped in a roctx range named after it (rocprofv3 --marker-trace); the
+ * marker library is opened at run time. */
 int p2r_create(int device, p2r_ctx** ctx);
 int p2r_destroy(p2r_ctx* ctx);
 p2r_params p2r_default_params(void);

@@ -1095,6 +1095,7 @@ int p2r_create(int device, p2r_ctx** out) {
     return P2R_ERR_HIP;
   }
   c->num_cus = prop.multiProcessorCount;
+  c->roctx.load();
   *out = c;
   return P2R_OK;
 }
@@ -1115,6 +1116,7 @@ const char* p2r_last_error(const p2r_ctx* c) { return c ? c->msg : "null context
 int p2r_forward(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* s, int semantics,
                 const p2r_params* params, float* chi2, void* stream) {
   if (!c) return P2R_ERR_ARG;
+  RangeScope range(c, "p2r_forward");
   if (!trk_in || !hits || !trk_out || !s) return c->fail(P2R_ERR_ARG, "null pointer argument");
   if (semantics != P2R_REFERENCE && semantics != P2R_CHAINED) return c->fail(P2R_ERR_ARG, "bad semantics %d", semantics);
   if (s->nevts < 0 || s->ntrks < 0 || s->nlayer < 1 || s->bsize < 1 || s->bsize > 1024)
@@ -1143,6 +1145,7 @@ int p2r_forward_streamed(p2r_ctx* c, const void* host_trk_in, const void* host_h
                          void* dev_trk_in, void* dev_hits, void* dev_trk_out, const p2r_shape* s, int semantics,
                          const p2r_params* params, int32_t nstreams) {
   if (!c) return P2R_ERR_ARG;
+  RangeScope range(c, "p2r_forward_streamed");
   if (!host_trk_in || !host_hits || !host_trk_out || !dev_trk_in || !dev_hits || !dev_trk_out || !s)
     return c->fail(P2R_ERR_ARG, "null pointer argument");
   if (semantics != P2R_REFERENCE && semantics != P2R_CHAINED) return c->fail(P2R_ERR_ARG, "bad semantics %d", semantics);
@@ -1194,6 +1197,7 @@ int p2r_forward_ragged(p2r_ctx* c, const void* trk_in, const void* hits, void* t
                        int32_t nevts, const uint32_t* layer_mask, int32_t nlayer, int32_t bsize, int semantics,
                        const p2r_params* params, float* chi2, int32_t tile_rounds, void* stream) {
   if (!c) return P2R_ERR_ARG;
+  RangeScope range(c, "p2r_forward_ragged");
   if (!trk_in || !hits || !trk_out || !blk_offsets || !layer_mask) return c->fail(P2R_ERR_ARG, "null pointer argument");
   if (semantics != P2R_REFERENCE && semantics != P2R_CHAINED) return c->fail(P2R_ERR_ARG, "bad semantics %d", semantics);
   if (nevts < 0 || nlayer < 1 || nlayer > 32 || bsize < 1 || bsize > 1024)
@@ -1235,6 +1239,7 @@ int p2r_forward_adjoint(p2r_ctx* c, const void* trk_in, const void* hits, void* 
                         const p2r_params* params, int grad_precision, int32_t checkpoint_stride, float* chi2,
                         float* grad_par, float* grad_hits, float* grad_kfact, double* grad_global, void* stream) {
   if (!c) return P2R_ERR_ARG;
+  RangeScope range(c, "p2r_forward_adjoint");
   AdjointPlan pl;
   const int rc = adjoint_plan(c, trk_in, hits, trk_out, s, params, grad_precision, checkpoint_stride,
                               grad_global != nullptr, pl);
@@ -1250,6 +1255,7 @@ int p2r_forward_adjoint(p2r_ctx* c, const void* trk_in, const void* hits, void* 
 int p2r_graph_forward(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* s,
                       int semantics, const p2r_params* params, float* chi2, p2r_graph** out) {
   if (!c) return P2R_ERR_ARG;
+  RangeScope range(c, "p2r_graph_forward");
   if (!out) return c->fail(P2R_ERR_ARG, "null graph pointer");
   *out = nullptr;
   return capture_graph(c, nullptr, out, [&](hipStream_t st) {
@@ -1262,6 +1268,7 @@ int p2r_graph_forward_ragged(p2r_ctx* c, const void* trk_in, const void* hits, v
                              int32_t bsize, int semantics, const p2r_params* params, float* chi2,
                              int32_t tile_rounds, p2r_graph** out) {
   if (!c) return P2R_ERR_ARG;
+  RangeScope range(c, "p2r_graph_forward_ragged");
   if (!out) return c->fail(P2R_ERR_ARG, "null graph pointer");
   *out = nullptr;
   return capture_graph(c, nullptr, out, [&](hipStream_t st) {
@@ -1275,6 +1282,7 @@ int p2r_graph_forward_adjoint(p2r_ctx* c, const void* trk_in, const void* hits, 
                               int32_t checkpoint_stride, float* chi2, float* grad_par, float* grad_hits,
                               float* grad_kfact, double* grad_global, p2r_graph** out) {
   if (!c) return P2R_ERR_ARG;
+  RangeScope range(c, "p2r_graph_forward_adjoint");
   if (!out) return c->fail(P2R_ERR_ARG, "null graph pointer");
   *out = nullptr;
   AdjointPlan pl;
@@ -1321,6 +1329,7 @@ int p2r_propagateToR(p2r_ctx* c, const float* inErr, const float* inPar, const i
                      float* outErr, float* outPar, int32_t bsize, int64_t nblk, int64_t trk_stride,
                      int64_t hit_stride, float kfact, void* stream) {
   if (!c) return P2R_ERR_ARG;
+  RangeScope range(c, "p2r_propagateToR");
   if (!inErr || !inPar || !inChg || !msP || !outErr || !outPar) return c->fail(P2R_ERR_ARG, "null pointer argument");
   if (bsize < 1 || bsize > 1024 || nblk < 0 || trk_stride < 0 || hit_stride < 0)
     return c->fail(P2R_ERR_SHAPE, "bad bsize/nblk/stride");
@@ -1336,6 +1345,7 @@ int p2r_propagateToR(p2r_ctx* c, const float* inErr, const float* inPar, const i
 int p2r_KalmanUpdate(p2r_ctx* c, float* trkErr, float* inPar, const float* hitErr, const float* msP, int32_t bsize,
                      int64_t nblk, int64_t trk_stride, int64_t hit_stride, void* stream) {
   if (!c) return P2R_ERR_ARG;
+  RangeScope range(c, "p2r_KalmanUpdate");
   if (!trkErr || !inPar || !hitErr || !msP) return c->fail(P2R_ERR_ARG, "null pointer argument");
   if (bsize < 1 || bsize > 1024 || nblk < 0 || trk_stride < 0 || hit_stride < 0)
     return c->fail(P2R_ERR_SHAPE, "bad bsize/nblk/stride");

@@ -2,8 +2,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 
 #include "../../include/p2r.h"
 
@@ -20,8 +24,27 @@ struct DeviceGuard {
   }
 };
 
+// Optional roctx ranges around the entry points (recorded by `rocprofv3 --marker-trace`), enabled with
+// P2R_ROCTX=1 in the environment at p2r_create.  The marker library is opened at run time, so the
+// library does not link against it and costs nothing when the ranges are off.
+struct Roctx {
+  int (*push)(const char*) = nullptr;
+  int (*pop)() = nullptr;
+  void* handle = nullptr;
+  void load() {
+    const char* e = std::getenv("P2R_ROCTX");
+    if (!e || std::strcmp(e, "1") != 0) return;
+    handle = dlopen("librocprofiler-sdk-roctx.so.1", RTLD_NOW | RTLD_LOCAL);
+    if (!handle) return;
+    push = reinterpret_cast<int (*)(const char*)>(dlsym(handle, "roctxRangePushA"));
+    pop = reinterpret_cast<int (*)()>(dlsym(handle, "roctxRangePop"));
+    if (!push || !pop) push = nullptr, pop = nullptr;
+  }
+};
+
 struct p2r_ctx {
   int device = 0;
+  Roctx roctx;
   int num_cus = 0;
   char msg[256] = "no error";
   // Launch choices (p2r_set_tuning / the autotuners); defaults below.
@@ -72,5 +95,16 @@ struct p2r_ctx {
     if (ws) (void)hipFree(ws);
     ws = nullptr;
     ws_bytes = 0;
+  }
+};
+
+// A roctx range for the scope of one entry point (no-op when the context's ranges are off).
+struct RangeScope {
+  const Roctx* r;
+  RangeScope(const p2r_ctx* c, const char* name) : r(c && c->roctx.push ? &c->roctx : nullptr) {
+    if (r) r->push(name);
+  }
+  ~RangeScope() {
+    if (r) r->pop();
   }
 };

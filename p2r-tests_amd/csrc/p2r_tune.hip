@@ -140,6 +140,7 @@ int p2r_autotune_forward(p2r_ctx* c, const void* trk_in, const void* hits, void*
                          int semantics, const p2r_params* params, int32_t reps, p2r_tune_report* report,
                          void* stream) {
   if (!c) return P2R_ERR_ARG;
+  RangeScope range(c, "p2r_autotune_forward");
   if (!s || reps < 1) return c->fail(P2R_ERR_ARG, "null shape or reps < 1");
   const p2r_tuning keep = c->tune;
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -192,6 +193,7 @@ int p2r_autotune_ragged(p2r_ctx* c, const void* trk_in, const void* hits, void* 
                         int32_t nevts, const uint32_t* layer_mask, int32_t nlayer, int32_t bsize, int semantics,
                         const p2r_params* params, int32_t reps, p2r_tune_report* report, void* stream) {
   if (!c) return P2R_ERR_ARG;
+  RangeScope range(c, "p2r_autotune_ragged");
   if (reps < 1) return c->fail(P2R_ERR_ARG, "reps < 1");
   const p2r_tuning keep = c->tune;
   hipStream_t st = static_cast<hipStream_t>(stream);
@@ -234,6 +236,7 @@ int p2r_autotune_adjoint(p2r_ctx* c, const void* trk_in, const void* hits, void*
                          float* grad_hits, float* grad_kfact, double* grad_global, int32_t reps,
                          p2r_tune_report* report, void* stream) {
   if (!c) return P2R_ERR_ARG;
+  RangeScope range(c, "p2r_autotune_adjoint");
   if (!s || reps < 1) return c->fail(P2R_ERR_ARG, "null shape or reps < 1");
   hipStream_t st = static_cast<hipStream_t>(stream);
   const int strides[4] = {1, 2, 4, 8};
@@ -282,6 +285,7 @@ int p2r_autotune_width(p2r_ctx* c, const void* trk_in, const void* hits, const p
                        const p2r_params* params, const int32_t* widths, int32_t nwidths, int32_t reps,
                        p2r_tune_report* report, void* stream) {
   if (!c) return P2R_ERR_ARG;
+  RangeScope range(c, "p2r_autotune_width");
   if (!s || !widths || nwidths < 1 || nwidths > 32 || reps < 1)
     return c->fail(P2R_ERR_ARG, "null shape/widths, nwidths outside 1..32 or reps < 1");
   for (int i = 0; i < nwidths; ++i)

@@ -22,12 +22,12 @@ EXE = os.path.join(ROOT, "p2r-tests_amd", "lib", "p2r_bench")
 GOLDEN = os.path.join(ROOT, "tests", "golden", "c0_summary_ref.txt")
 
 
-def _run(args):
+def _run(args, env=None):
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     assert os.path.exists(EXE), f"driver not built: {EXE}"
     p = subprocess.run([EXE] + args, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, universal_newlines=True,
-                       timeout=120)
+                       timeout=120, env=None if env is None else dict(os.environ, **env))
     assert p.returncode == 0, p.stdout
     return p.stdout
 
@@ -78,3 +78,10 @@ def test_driver_baseline_inputs_and_transfers():
     sums = [_summary(o) for o in outs]
     assert all(s == sums[0] for s in sums[1:])
     assert all("done ntracks=38400 " in o for o in outs)
+
+
+def test_driver_with_roctx_ranges_enabled():
+    """P2R_ROCTX=1 opens the marker library and wraps every entry point in a range; outside a profiler the
+    ranges are no-ops and the run prints the same summary."""
+    args = ["--input", "baseline", "--nevts", "2", "--NITER", "1"]
+    assert _summary(_run(args, {"P2R_ROCTX": "1"})) == _summary(_run(args))
