@@ -190,6 +190,17 @@ def test_fast_baseline_config1(orc, orc_alt, ctx, semantics, mode):
     check_fast(orc, orc_alt, ctx, trk, hits, shape, semantics, f"config1-mode{mode}")
 
 
+@pytest.mark.parametrize("semantics", ["reference", "chained"])
+def test_fast_full_headline_shape(orc, orc_alt, ctx, semantics):
+    """The fast path on the whole 100 x 9600 x 20 batch (the benchmark's inputs), by the same statistical
+    criterion as the config-1 tests: no worse against the strict oracle than the reference's own fast CPU
+    build is, and close to the FP64 replay."""
+    import p2r_amd
+    shape = Shape(100, 9600, 20, 32)
+    trk, hits = p2r_amd.gen_uniform(shape)
+    check_fast(orc, orc_alt, ctx, trk, hits, shape, semantics, "full")
+
+
 # ---- errors and work accounting ---------------------------------------------
 
 def test_shape_errors(ctx):
