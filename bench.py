@@ -441,25 +441,39 @@ def measured_traffic():
 
 
 def cpu_baseline(shape, trk, hits, semantics, seconds):
-    """The oracle's restatement of the reference CPU path (port), OpenMP over (event, block), on a bounded
-    sample (leading events of the same inputs), repeated until `seconds` of wall time."""
+    """The reference CPU path on this host's cores, on a bounded sample (leading events of the same inputs),
+    each leg repeated until `seconds` of wall time.  Kind "reference": the reference's own acc.v3 block
+    routines compiled with its CPU flags (-O3 -ffast-math) in tbb.cpp's driver loop, OpenMP over Matriplex
+    blocks (oracle/ref_acc_bench.cpp; reference semantics, bsize 32).  Kind "port": the oracle's
+    restatement with the same flags, used when that build is absent or for chained semantics; its rate is
+    reported beside the reference's either way."""
     import oracle
     from p2r_amd import Shape
     threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
-    orc = oracle.Oracle("liboracle_bench.so")
     nev = min(shape.nevts, 10)
     s = Shape(nev, shape.ntrks, shape.nlayer, shape.bsize)
     t1, h1 = trk[: s.trk_floats], hits[: s.hit_floats]
+
+    def timed(run):
+        run()  # warm-up
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            run()
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= seconds or reps >= 1000:
+                return reps, el, reps * s.ntrk_total * s.nlayer / el
+
+    orc = oracle.Oracle("liboracle_bench.so")
     sem = 0 if semantics == "reference" else 1
-    orc.run_f32(t1, h1, s.nblk, s.nlayer, s.bsize, semantics=sem, nthreads=threads)  # warm-up
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        orc.run_f32(t1, h1, s.nblk, s.nlayer, s.bsize, semantics=sem, nthreads=threads)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or reps >= 1000:
-            break
-    rate = reps * s.ntrk_total * s.nlayer / el
+    preps, pel, prate = timed(lambda: orc.run_f32(t1, h1, s.nblk, s.nlayer, s.bsize, semantics=sem, nthreads=threads))
+    kind, reps, el, rate = "port", preps, pel, prate
+    how = "the oracle's restatement of tbb.cpp"
+    if semantics == "reference" and oracle.ref_bench_available():
+        rb = oracle.RefBench()
+        if rb.bs == s.bsize:
+            reps, el, rate = timed(lambda: rb.run(t1, h1, s.nblk, s.nlayer, threads))
+            kind, how = "reference", "the reference's acc.v3.cpp block routines in tbb.cpp's driver loop"
     model = ""
     try:
         for l in open("/proc/cpuinfo"):
@@ -468,10 +482,11 @@ def cpu_baseline(shape, trk, hits, semantics, seconds):
                 break
     except OSError:
         pass
-    return {"value": rate, "unit": "track-layer updates/s", "cores": threads, "kind": "port",
+    return {"value": rate, "unit": "track-layer updates/s", "cores": threads, "kind": kind,
             "sample": f"{nev} events x {shape.ntrks} tracks x {shape.nlayer} layers (leading events of the "
-                      f"benchmark inputs), {reps} repetitions in {el:.1f} s, OpenMP over Matriplex blocks, "
-                      f"-O3 -ffast-math -march=x86-64-v3",
+                      f"benchmark inputs), {reps} repetitions in {el:.1f} s: {how}, OpenMP over Matriplex "
+                      f"blocks, -O3 -ffast-math -march=x86-64-v3",
+            "port_value": prate, "port_sample": f"{preps} repetitions in {pel:.1f} s",
             "cpu_model": model, "semantics": semantics}
 
 

@@ -240,3 +240,24 @@ class Ref:
 
 def ref_available():
     return all(os.path.exists(os.path.join(HERE, "_ref", n)) for n in ("libref_acc.so", "libref_pstl.so", "pstl_c0"))
+
+
+class RefBench:
+    """The reference's acc.v3 block routines built with its CPU flags and driven like tbb.cpp's loop
+    (oracle/ref_acc_bench.cpp, output _ref/libref_acc_bench.so): the CPU baseline of kind "reference"."""
+
+    NAME = "libref_acc_bench.so"
+
+    def __init__(self, name=None):
+        self.lib = C.CDLL(os.path.join(HERE, "_ref", name or self.NAME))
+        self.lib.ref_acc_bench_run.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_int, C.c_int]
+        self.bs = self.lib.ref_acc_bench_bsize()
+
+    def run(self, trk, hits, nblk, nlayer, nthreads=0):
+        out = np.zeros_like(trk)
+        assert self.lib.ref_acc_bench_run(_ptr(trk), _ptr(hits), _ptr(out), nblk, nlayer, nthreads) == 0
+        return out
+
+
+def ref_bench_available():
+    return os.path.exists(os.path.join(HERE, "_ref", RefBench.NAME))

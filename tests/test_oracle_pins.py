@@ -343,3 +343,25 @@ def test_masked_run_is_dense_run_on_compacted_hits(orc):
         assert np.array_equal(_bits(masked), _bits(dense)), present
     none = orc.run_f32(trk, hits, shape.nblk, 20, 32, semantics=1, mask=np.zeros(shape.ntrk_total, np.uint32))
     assert np.array_equal(_bits(none), _bits(trk))  # no present layer: copied through
+
+
+def test_reference_driver_harness(orc, ref):
+    """oracle/ref_acc_bench.cpp runs the reference's compiled block routines in tbb.cpp's driver loop (the
+    CPU baseline).  Built strict, it equals the oracle's reference-semantics run word for word (with the
+    acc.v3 kfact), which pins the driver loop; the timed build (-O3 -ffast-math) agrees in charge and
+    stays finite."""
+    import oracle
+    import p2r_amd
+    from p2r_amd import Shape
+    if not oracle.ref_bench_available():
+        pytest.skip("oracle/_ref timing harness not built")
+    shape = Shape(2, 640, 20, 32)
+    trk, hits = p2r_amd.gen_uniform(shape)
+    kf = ref.kfact
+    strict = oracle.RefBench("libref_acc_drive.so").run(trk, hits, shape.nblk, 20, 2)
+    want = orc.run_f32(trk, hits, shape.nblk, 20, 32, semantics=0, kfact=kf)
+    assert np.array_equal(_bits(strict), _bits(want))
+    fast = oracle.RefBench().run(trk, hits, shape.nblk, 20, 2)
+    f, w = fast.reshape(-1, 28, 32), want.reshape(-1, 28, 32)
+    assert np.array_equal(f[:, 27].view(np.int32), w[:, 27].view(np.int32))
+    assert np.isfinite(f[:, :27]).all()
