@@ -255,9 +255,9 @@ typedef struct p2r_tuning {
   int32_t ragged_threads;        /* workgroup size of the layer-ordered schedule: 64, 128 or 256 (256) */
   int32_t checkpoint_stride;     /* what checkpoint_stride = 0 means in p2r_forward_adjoint: 1, 2, 4, 8 (4) */
   int32_t fwd_waves;             /* register budget of the p2r_forward kernels, in waves per SIMD:
-                                    0 = the library's choice (default: 3 for P2R_EXACT, the
-                                    compiler's allocation for P2R_FAST), 1 = the compiler's
-                                    allocation, 3 or 4 (fewer registers, possibly spills) */
+                                    0 = the library's choice (default: 3 for P2R_EXACT, 4 for
+                                    P2R_FAST), 1 = the compiler's allocation, 3 or 4 (fewer
+                                    registers, possibly spills) */
 } p2r_tuning;
 int p2r_get_tuning(const p2r_ctx* ctx, p2r_tuning* out);
 int p2r_set_tuning(p2r_ctx* ctx, const p2r_tuning* in); /* P2R_ERR_ARG on a value outside the lists above */

@@ -93,8 +93,17 @@ __global__ __launch_bounds__(kThreads, MINW) void fwd_uniform_kernel(const float
     if (l + 1 < l1) load_hit(hb + (l + 1) * hstride, bs, hn);
     Trk src;
     if (SEM == P2R_REFERENCE) {
-      src = in;      // tbb.cpp:894: every layer starts from the input track
-      opaque(src);   // recompute per layer; nothing hoisted across layers
+      // tbb.cpp:894: every layer starts from the input track.  The barrier makes every layer recompute
+      // (nothing hoisted across layers): on a copy in the exact instance, on the input itself (in place,
+      // its values never change) in the fast one, where the saved registers allow the 4-wave budget.
+      // Each form measured the faster for its instance.
+      if (EXACT) {
+        src = in;
+        opaque(src);
+      } else {
+        opaque(in);
+        src = in;
+      }
       q = q_in;
       k = (float)q * kfact;
     } else {
@@ -162,8 +171,8 @@ __device__ __forceinline__ void ragged_slot(const float* __restrict__ trk, const
     if (m) load_hit(hb + (size_t)__builtin_ctz(m) * hstride, bs, hn);
     Trk src;
     if (SEM == P2R_REFERENCE) {
+      opaque(in);  // in place, no copy: recompute per layer
       src = in;
-      opaque(src);
       q = q_in;
       k = (float)q * kfact;
     } else {
@@ -311,8 +320,8 @@ __global__ __launch_bounds__(kThreads) void fwd_ragged_layer_kernel(const float*
     if ((m >> l) & 1u) {
       Trk src;
       if (SEM == P2R_REFERENCE) {
+        opaque(in);  // in place, no copy: recompute per layer
         src = in;
-        opaque(src);
         q = q_in;
         k = (float)q * kfact;
       } else {
@@ -822,9 +831,9 @@ void launch_uniform_w(dim3 grid, dim3 block, hipStream_t st, const UniformLaunch
 template <int SEM, bool CHI2, bool EXACT>
 void launch_uniform_e(int waves, dim3 grid, dim3 block, hipStream_t st, const UniformLaunch& a, const float* in,
                       int l0, int l1, bool acc) {
-  // 0 = the library's choice: the 3-wave budget for the exact instance (faster on MI355X in both
-  // semantics, measured), the compiler's allocation for the fast one
-  if (waves == 0) waves = EXACT ? 3 : 1;
+  // 0 = the library's choice: the 3-wave budget for the exact instance, 4 waves for the fast one (each
+  // the fastest measured on MI355X in both semantics)
+  if (waves == 0) waves = EXACT ? 3 : 4;
   if (waves == 3) launch_uniform_w<SEM, CHI2, EXACT, 3>(grid, block, st, a, in, l0, l1, acc);
   else if (waves == 4) launch_uniform_w<SEM, CHI2, EXACT, 4>(grid, block, st, a, in, l0, l1, acc);
   else launch_uniform_w<SEM, CHI2, EXACT, 1>(grid, block, st, a, in, l0, l1, acc);
