@@ -8,9 +8,10 @@ before the timed region; each step is one p2r_forward launch.
 
 Prints ONE JSON line.  value = track-layer updates/s (nevts*ntrks*nlayer per
 step / time per step).  `roofline` places the kernel against the FP32 vector
-peak (the kernel is VALU-bound), `cpu_baseline` times the
-oracle's restatement of the reference CPU path on this host's cores on a
-bounded sample of the same workload.
+peak (the kernel is VALU-bound: 1212 FLOP against 47.2 HBM bytes per update),
+`cpu_baseline` times the reference's own CPU kernel (compiled from its sources
+into oracle/_ref, OpenMP over blocks) on this host's cores on a bounded sample
+of the same workload, or the oracle's restatement where that library is absent.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--semantics reference|chained]
                   [--cpu-seconds S] [--no-cpu-baseline]
@@ -67,8 +68,8 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: this build runs the path on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not launched).", file=sys.stderr)
+        print("bench.py measures one GPU only: the p2r path runs on a single MI355X "
+              "(events would shard across GPUs with no communication; that path is not launched).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -31,7 +31,7 @@ typedef enum p2r_status {
   P2R_ERR_ARG = 1,         /* null pointer / invalid enum value */
   P2R_ERR_SHAPE = 2,       /* sizes the path does not accept (e.g. ntrks % bsize != 0) */
   P2R_ERR_HIP = 3,         /* a HIP runtime call failed; see p2r_last_error */
-  P2R_ERR_UNSUPPORTED = 4, /* valid request this build does not implement */
+  P2R_ERR_UNSUPPORTED = 4, /* valid request the library does not implement */
   P2R_ERR_NOMEM = 5        /* workspace allocation failed */
 } p2r_status;
 
