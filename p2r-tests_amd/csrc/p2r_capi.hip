@@ -375,40 +375,43 @@ __device__ __forceinline__ void wave_lds_sync() {
 // One step of the schedule from window start `c` (position = layer*kPsTile + slot).
 // Returns this lane's pair code (slot | layer << 7 | last-layer flag << 12 |
 // lane-in-block << 13 | block - the tile's first block << 23) or kPsNone, and
-// the next window start in `next` (wave-uniform).  m, rb, ln: mask, relative
-// block and lane-in-block of this lane's two slots.
-__device__ __forceinline__ uint32_t ps_step(uint32_t c, const uint32_t m[2], const uint32_t rb[2], const uint32_t ln[2],
-                                           int lane, uint32_t* tbl, uint32_t& next) {
-  const uint32_t L0 = c >> 7, s0 = c & (kPsTile - 1);
-  uint64_t p1[2], p2[2];
-  bool bit[2], first[2];
-  int L[2];
+// the next window start in `next` (wave-uniform).  m: mask of this lane's two
+// slots (lane, lane + 64); sc: their constant code bits (slot, lane-in-block,
+// relative block).
+__device__ __forceinline__ uint32_t ps_step(uint32_t c, const uint32_t m[2], const uint32_t sc[2], int lane,
+                                           uint32_t* tbl, uint32_t& next) {
+  c = __builtin_amdgcn_readfirstlane(c);  // wave-uniform: keeps the window arithmetic scalar
+  const uint32_t L0 = c >> 7, s0 = c & (kPsTile - 1);  // L0 < nlayer <= 32
+  // Window order: slots s0..127 at layer L0, then slots 0..s0-1 at layer L0 + 1.  A slot is present
+  // in the window if its mask has the layer the window gives it.
+  bool first[2], pres[2];
+  uint64_t P[2];
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
-    const uint32_t slot = (uint32_t)lane + 64u * r;
-    first[r] = slot >= s0;
-    L[r] = first[r] ? (int)L0 : (int)L0 + 1;
-    bit[r] = L[r] < 32 && ((m[r] >> L[r]) & 1u);
-    p1[r] = __ballot(bit[r] && first[r]);
-    p2[r] = __ballot(bit[r] && !first[r]);
+    first[r] = (uint32_t)lane + 64u * r >= s0;
+    pres[r] = ((m[r] >> L0) >> (first[r] ? 0u : 1u)) & 1u;
+    P[r] = __ballot(pres[r]);
   }
-  const uint32_t n1a = __popcll(p1[0]), n1 = n1a + __popcll(p1[1]);
-  const uint32_t n2a = __popcll(p2[0]), total = n1 + n2a + __popcll(p2[1]);
-  const uint64_t below = (1ull << lane) - 1ull;
+  const uint32_t c0 = __popcll(P[0]), total = c0 + __popcll(P[1]);
+  const uint32_t before_s0 =  // present slots in [0, s0): the window's layer-(L0 + 1) part
+      s0 < 64u ? __popcll(P[0] & ((1ull << s0) - 1ull)) : c0 + __popcll(P[1] & ((1ull << (s0 - 64u)) - 1ull));
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
-    if (!bit[r]) continue;
-    const uint32_t rank = first[r] ? (r ? n1a : 0u) + __popcll(p1[r] & below)
-                                   : n1 + (r ? n2a : 0u) + __popcll(p2[r] & below);
-    if (rank <= 64) {
-      const uint32_t last = (m[r] >> L[r]) <= 1u;  // no present layer above L
-      tbl[rank] = ((uint32_t)lane + 64u * r) | ((uint32_t)L[r] << 7) | (last << 12) | (ln[r] << 13) | (rb[r] << 23);
+    // present slots in [0, slot), then the position in window order (mod 2^32: the second part wraps)
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(P[r] >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)P[r], r ? c0 : 0u));
+    const uint32_t rank = below - before_s0 + (first[r] ? 0u : total);
+    if (pres[r] && rank <= 64) {
+      const uint32_t L = L0 + (first[r] ? 0u : 1u);  // a present layer, so L < nlayer <= 32
+      const uint32_t last = (m[r] >> L) <= 1u;       // no present layer above L
+      tbl[rank] = sc[r] | (L << 7) | (last << 12);
     }
   }
   wave_lds_sync();
   const uint32_t take = total < 64 ? total : 64u;
   const uint32_t code = (uint32_t)lane < take ? tbl[lane] : kPsNone;
-  next = total > 64 ? (__builtin_amdgcn_readfirstlane(tbl[64]) & 0xfffu) : c + kPsTile;
+  const uint32_t t64 = tbl[64];
+  next = __builtin_amdgcn_readfirstlane(total > 64 ? (t64 & 0xfffu) : c + kPsTile);
   wave_lds_sync();
   return code;
 }
@@ -425,21 +428,21 @@ __global__ __launch_bounds__(64 * kPsWaves) void fwd_ragged_stream_kernel(const 
   const bool per_layer = opts & P2R_OPT_CHI2_PER_LAYER;
   __shared__ float st_all[kPsWaves][kW][kPsTile];
   __shared__ uint32_t tbl_all[kPsWaves][65];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;  // wv: wave-uniform
   const uint32_t base = (blockIdx.x * kPsWaves + wv) * (uint32_t)kPsTile;
   if (base >= nslot) return;  // whole wavefront; no workgroup barriers below
   float(*st)[kPsTile] = st_all[wv];
   uint32_t* tbl = tbl_all[wv];
   const uint32_t layer_bits = nlayer >= 32 ? ~0u : ((1u << nlayer) - 1u);
-  const uint32_t blk0 = base / bs;  // the tile's blocks are blk0 .. blk0 + 127 at most
-  uint32_t m[2], b[2], rb[2], ln[2];
+  const uint32_t blk0 = __builtin_amdgcn_readfirstlane(base / bs);  // the tile's blocks: blk0 .. blk0 + 128 at most
+  uint32_t m[2], b[2], ln[2], sc[2];
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
     const uint32_t slot = (uint32_t)lane + 64u * r, g = base + slot;
     m[r] = 0u;
     b[r] = g / bs;
-    rb[r] = b[r] - blk0;
     ln[r] = g - b[r] * bs;
+    sc[r] = slot | (ln[r] << 13) | ((b[r] - blk0) << 23);
     if (g < nslot) {
       const float* tb = trk + (size_t)b[r] * 28 * bs + ln[r];
 #pragma unroll
@@ -449,24 +452,30 @@ __global__ __launch_bounds__(64 * kPsWaves) void fwd_ragged_stream_kernel(const 
     }
   }
   wave_lds_sync();
-  const size_t hstride = (size_t)9 * bs;
-  auto hit_ptr = [&](uint32_t code) {
-    const uint32_t L = (code >> 7) & 31u;
-    return hits + ((size_t)(blk0 + (code >> 23)) * nlayer + L) * hstride + ((code >> 13) & 1023u);
+  // Hits of the tile's blocks through a buffer resource over their rows: a 32-bit byte offset per
+  // pair, the field stride as the scalar offset.  The tile spans at most 127 / bs + 2 blocks, under
+  // 2.4 MB of hit rows for bs <= 1024 and nlayer <= 32; the record count is that span, so an offset
+  // past it would read 0 rather than another tile's memory.
+  const uint32_t row_bytes = 36u * bs, field_bytes = 4u * bs;
+  const uint32_t blk_last = (base + kPsTile - 1 < nslot ? base + kPsTile - 1 : nslot - 1) / bs;
+  const uint64_t hrow = reinterpret_cast<uint64_t>(hits + (size_t)blk0 * nlayer * 9 * bs);
+  // readfirstlane returns int: each half goes back through uint32_t so the low word is not sign-extended
+  const uint64_t hrow_u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(hrow >> 32)) << 32) |
+                          (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)hrow);  // wave-uniform
+  const __amdgpu_buffer_rsrc_t hrs = __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(hrow_u), (short)0,
+      (int)__builtin_amdgcn_readfirstlane((blk_last - blk0 + 1) * (uint32_t)nlayer * row_bytes), 0x00020000);
+  auto load_pair_hit = [&](uint32_t code, Hit& h) {
+    const uint32_t off = ((code >> 23) * (uint32_t)nlayer + ((code >> 7) & 31u)) * row_bytes + ((code >> 11) & 0xffcu);
+    auto ld = [&](int f) {
+      return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                           hrs, off, __builtin_amdgcn_readfirstlane(f * field_bytes), 0));
+    };
+#pragma unroll
+    for (int f = 0; f < 3; ++f) h.pos[f] = ld(f);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) h.cov[f] = ld(3 + f);
   };
-  const uint32_t cend = (uint32_t)nlayer * kPsTile;
-  uint32_t c = 0, cn = 0;
-  uint32_t code_n = ps_step(c, m, rb, ln, lane, tbl, cn);
-  Hit hn;
-  if (code_n != kPsNone) load_hit_cached(hit_ptr(code_n), bs, hn);
-  while (c < cend) {
-    const uint32_t code = code_n;
-    const Hit h = hn;
-    c = cn;
-    if (c < cend) {  // next step's pairs and hit loads, issued before this step's arithmetic
-      code_n = ps_step(c, m, rb, ln, lane, tbl, cn);
-      if (code_n != kPsNone) load_hit_cached(hit_ptr(code_n), bs, hn);
-    }
+  auto process = [&](uint32_t code, const Hit& h) {
     if (code != kPsNone) {
       const uint32_t slot = code & (kPsTile - 1);
       Trk src;
@@ -502,6 +511,29 @@ __global__ __launch_bounds__(64 * kPsWaves) void fwd_ragged_stream_kernel(const 
       }
     }
     wave_lds_sync();
+  };
+  // Steps alternate between two hit registers sets: the next step's pairs and hit loads are issued
+  // before the current step's arithmetic, with no copies between them.
+  const uint32_t cend = (uint32_t)nlayer * kPsTile;
+  uint32_t cn = 0;
+  uint32_t code_a = ps_step(0u, m, sc, lane, tbl, cn), code_b = kPsNone;
+  Hit ha, hb;
+  if (code_a != kPsNone) load_pair_hit(code_a, ha);
+  for (;;) {
+    const bool more_b = cn < cend;
+    if (more_b) {
+      code_b = ps_step(cn, m, sc, lane, tbl, cn);
+      if (code_b != kPsNone) load_pair_hit(code_b, hb);
+    }
+    process(code_a, ha);
+    if (!more_b) break;
+    const bool more_a = cn < cend;
+    if (more_a) {
+      code_a = ps_step(cn, m, sc, lane, tbl, cn);
+      if (code_a != kPsNone) load_pair_hit(code_a, ha);
+    }
+    process(code_b, hb);
+    if (!more_a) break;
   }
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
