@@ -58,6 +58,45 @@ __device__ __forceinline__ void load_hit_cached(const float* hb, uint32_t bs, Hi
   for (int f = 0; f < 6; ++f) h.cov[f] = hb[(size_t)(3 + f) * bs];
 }
 
+// Hit rows of the blocks a wavefront's lanes belong to, read through a buffer resource: each lane's
+// byte offset is fixed for the whole pass and the layer and field go in the scalar offset, so a
+// layer's 9 loads need no address arithmetic.  The active lanes of a wavefront are a prefix of its
+// 64 consecutive tracks (only the batch's end cuts it), so they span at most 63 / bs + 2 blocks:
+// under 2.4 MB of hit rows for bs <= 1024 and nlayer <= 32.  The record count is that span.
+struct WaveHits {
+  __amdgpu_buffer_rsrc_t rs;
+  uint32_t voff, field_bytes;
+  __device__ __forceinline__ WaveHits(const float* hits, uint32_t g, uint32_t ntrk, uint32_t bs, int nlayer,
+                                      uint32_t blk, uint32_t lane) {
+    const uint32_t g0 = __builtin_amdgcn_readfirstlane(g);  // the wavefront's first track
+    const uint32_t blk0 = __builtin_amdgcn_readfirstlane(g0 / bs);
+    const uint32_t blk1 = (g0 + 63u < ntrk ? g0 + 63u : ntrk - 1u) / bs;
+    const uint32_t row_bytes = 36u * bs;
+    field_bytes = 4u * bs;
+    const uint64_t a = reinterpret_cast<uint64_t>(hits + (size_t)blk0 * nlayer * 9 * bs);
+    // readfirstlane returns int: each half goes back through uint32_t so the low word is not sign-extended
+    const uint64_t au = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32)) << 32) |
+                        (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
+    rs = __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(au), (short)0,
+                                           (int)__builtin_amdgcn_readfirstlane((blk1 - blk0 + 1u) * (uint32_t)nlayer *
+                                                                               row_bytes),
+                                           0x00020000);
+    voff = ((blk - blk0) * (uint32_t)nlayer * 9u * bs + lane) * 4u;
+  }
+  // Layer l's hit, streamed (non-temporal: each hit is read once).
+  __device__ __forceinline__ void load(int l, Hit& h) const {
+    auto ld = [&](int f) {
+      return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                           rs, voff, __builtin_amdgcn_readfirstlane((uint32_t)(l * 9 + f) * field_bytes),
+                                           2 /* nt */));
+    };
+#pragma unroll
+    for (int f = 0; f < 3; ++f) h.pos[f] = ld(f);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) h.cov[f] = ld(3 + f);
+  }
+};
+
 // Whole batch, all layers per lane.  SEM: P2R_REFERENCE / P2R_CHAINED;
 // EXACT selects the per-lane math instance (p2r_lane.hpp).
 // Layers [l0, l1) of nlayer (all of them for one launch; a layer range when the
@@ -80,8 +119,7 @@ __global__ __launch_bounds__(kThreads, MINW) void fwd_uniform_kernel(const float
   int q = q_in;
   float k = (float)q * kfact;  // tbb.cpp:667
 
-  const float* hb = hits + (size_t)blk * nlayer * 9 * bs + lane;
-  const size_t hstride = (size_t)9 * bs;
+  const WaveHits wh(hits, g, ntrk, bs, nlayer, blk, lane);
   Trk cur = in;
   const bool per_layer = opts & P2R_OPT_CHI2_PER_LAYER;
   // a later launch of a split layer loop continues the running sum (same rounding as one launch)
@@ -92,7 +130,7 @@ __global__ __launch_bounds__(kThreads, MINW) void fwd_uniform_kernel(const float
   // extra waves hide the load latency: measured 5-7 % faster.
   for (int l = l0; l < l1; ++l) {
     Hit h;
-    load_hit(hb + (size_t)l * hstride, bs, h);
+    wh.load(l, h);
     Trk src;
     if (SEM == P2R_REFERENCE) {
       // tbb.cpp:894: every layer starts from the input track.  The barrier, on the input itself (in place,
