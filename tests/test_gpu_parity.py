@@ -291,6 +291,21 @@ def test_exact_ragged_other_widths(orc, ctx, bsize, rounds):
     assert n == 0 and bit_mismatches(c2g, c2c) == 0
 
 
+@pytest.mark.parametrize("rounds", [-1, -2])
+@pytest.mark.parametrize("semantics", ["chained", "reference"])
+def test_exact_ragged_widest_matriplex(orc, ctx, semantics, rounds):
+    """bsize 1024 (the largest width): a pair-stream tile lies inside one or two blocks, the hit offsets and
+    field strides are at their largest, and events end mid-block."""
+    import p2r_amd
+    counts = np.array([1500, 0, 2100, 700], np.int32)
+    trk, hits, mask, offs = p2r_amd.gen_ragged(counts, 20, 1024, kmin=5, kmax=20)
+    nblk = int(offs[-1])
+    gpu, c2g = run_ragged(ctx, trk, hits, mask, offs, 20, 1024, semantics, rounds=rounds)
+    cpu, c2c = orc.run_f32(trk, hits, nblk, 20, 1024, semantics=0 if semantics == "reference" else 1, chi2=True,
+                           mask=mask, cr_trig=True)
+    assert bit_mismatches(gpu, cpu) == 0 and bit_mismatches(c2g, c2c) == 0
+
+
 def test_ragged_full_masks_equal_uniform(ctx):
     """All layers present and equal event sizes: the ragged launch reproduces p2r_forward bit for bit."""
     import p2r_amd
