@@ -1199,9 +1199,9 @@ int p2r_forward(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out,
   if (ntrk >= (1ull << 31)) return c->fail(P2R_ERR_SHAPE, "too many tracks (%llu)", (unsigned long long)ntrk);
   const p2r_params p = params ? *params : p2r_default_params();
   if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
-  if (ntrk == 0) return P2R_OK;
   if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
   if (p.options & ~kAllOptions) return c->fail(P2R_ERR_ARG, "bad options %d", p.options);
+  if (ntrk == 0) return P2R_OK;
   DeviceGuard g(c->device);
   const UniformLaunch a{static_cast<const float*>(trk_in), static_cast<const float*>(hits), static_cast<float*>(trk_out),
                         chi2, (uint32_t)ntrk, (uint32_t)s->bsize, s->nlayer, p.kfact / p.b_scale, p.options};
@@ -1233,9 +1233,18 @@ int p2r_forward_streamed(p2r_ctx* c, const void* host_trk_in, const void* host_h
   DeviceGuard g(c->device);
   const size_t bs = (size_t)s->bsize, tw = 28 * bs * 4, hw = (size_t)s->nlayer * 9 * bs * 4;  // bytes per block
   const int ns = (int)std::min<uint64_t>((uint64_t)nstreams, nblk);
+  // On an error, the chunks already queued are drained before returning, so no copy or launch of this call
+  // is still reading or writing the caller's buffers when it returns.
+  auto drain = [&](int upto) {
+    for (int k = 0; k < upto; ++k)
+      if (hipStream_t st = c->stream(k)) (void)hipStreamSynchronize(st);
+  };
   for (int k = 0; k < ns; ++k) {
     hipStream_t st = c->stream(k);
-    if (!st) return c->fail(P2R_ERR_HIP, "hipStreamCreateWithFlags failed");
+    if (!st) {
+      drain(k);
+      return c->fail(P2R_ERR_HIP, "hipStreamCreateWithFlags failed");
+    }
     const uint64_t b0 = nblk * k / ns, b1 = nblk * (k + 1) / ns;  // contiguous block range of this chunk
     if (b1 == b0) continue;
     const size_t nb = b1 - b0;
@@ -1244,18 +1253,25 @@ int p2r_forward_streamed(p2r_ctx* c, const void* host_trk_in, const void* host_h
     char* dh = static_cast<char*>(dev_hits) + b0 * hw;
     char* dout = static_cast<char*>(dev_trk_out) + b0 * tw;
     if (hipMemcpyAsync(dt, off(host_trk_in, b0 * tw), nb * tw, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(dh, off(host_hits, b0 * hw), nb * hw, hipMemcpyHostToDevice, st) != hipSuccess)
+        hipMemcpyAsync(dh, off(host_hits, b0 * hw), nb * hw, hipMemcpyHostToDevice, st) != hipSuccess) {
+      drain(k + 1);
       return c->fail(P2R_ERR_HIP, "host-to-device copy of chunk %d failed", k);
+    }
     const UniformLaunch a{reinterpret_cast<const float*>(dt), reinterpret_cast<const float*>(dh),
                           reinterpret_cast<float*>(dout), nullptr, (uint32_t)(nb * bs), (uint32_t)bs, s->nlayer,
                           p.kfact / p.b_scale, p.options};
     launch_uniform(semantics, p.precision == P2R_EXACT, c->tune.fwd_threads, c->tune.fwd_layers_per_launch,
                    c->tune.fwd_waves, st, a);
     const int rc = c->check_launch("p2r_forward_streamed");
-    if (rc != P2R_OK) return rc;
+    if (rc != P2R_OK) {
+      drain(k + 1);
+      return rc;
+    }
     if (hipMemcpyAsync(static_cast<char*>(host_trk_out) + b0 * tw, dout, nb * tw, hipMemcpyDeviceToHost, st) !=
-        hipSuccess)
+        hipSuccess) {
+      drain(k + 1);
       return c->fail(P2R_ERR_HIP, "device-to-host copy of chunk %d failed", k);
+    }
   }
   for (int k = 0; k < ns; ++k)
     if (hipStreamSynchronize(c->stream(k)) != hipSuccess) return c->fail(P2R_ERR_HIP, "stream %d failed", k);

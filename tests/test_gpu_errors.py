@@ -54,6 +54,8 @@ def test_forward_argument_errors(ctx):
     for prm in (p2r_amd.default_params(precision=7), p2r_amd.default_params(b_scale=0.0),
                 p2r_amd.default_params(options=8)):
         assert _status(lambda: ctx.forward(t, h, t, Shape(1, 64, 20, 32), params=prm)) == p2r_amd.P2R_ERR_ARG
+        # an empty batch is no excuse for an invalid request
+        assert _status(lambda: ctx.forward(t, h, t, Shape(0, 64, 20, 32), params=prm)) == p2r_amd.P2R_ERR_ARG
     ctx.forward(t, h, t, Shape(1, 64, 20, 32))  # still usable
 
 
