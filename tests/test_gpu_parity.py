@@ -139,6 +139,24 @@ def test_exact_nonfinite_inputs(orc, ctx):
         assert not np.isfinite(gpu.reshape(-1, 28, 32)[0, :27, 3]).all()
 
 
+def test_exact_reciprocal_range_edges(orc, ctx):
+    """Finite inputs at the edges of the reciprocal's range, one lane each in otherwise ordinary wavefronts:
+    1/pt = 2^126 (pt = 2^-126, the smallest normal), 1/pt = 3e38 (pt denormal), 1/pt denormal (pt overflows)
+    and 1/pt = 2^125.  Every word equals the oracle's, with and without chi2."""
+    import p2r_amd
+    shape = Shape(1, 256, 20, 32)
+    trk, hits = p2r_amd.gen_uniform(shape)
+    t = trk.reshape(shape.nblk, 28, 32)
+    t[0, 3, 5] = np.float32(2.0 ** 126)       # pt = 2^-126: the largest input outside the verified range
+    t[1, 3, 9] = np.float32(3.0e38)           # pt denormal
+    t[2, 3, 1] = np.float32(2.0e-39)          # 1/pt denormal -> pt overflows
+    t[3, 3, 30] = np.float32(2.0 ** 125)      # just inside the range
+    for sem in ("reference", "chained"):
+        _, cpu = check_exact(orc, ctx, trk, hits, shape, sem, "rcp range")
+        gpu, _ = run_gpu(ctx, trk, hits, shape, sem, "exact", chi2=False)  # the instance without chi2 as well
+        assert bit_mismatches(gpu, cpu) == 0
+
+
 def test_exact_block_entry_points(orc, ctx):
     """p2r_propagateToR / p2r_KalmanUpdate on MPTRK/MPHIT arrays vs the oracle's block routines."""
     import p2r_amd
