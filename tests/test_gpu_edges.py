@@ -1,6 +1,6 @@
-"""GPU edge cases of the C-ABI: layer counts other than 20 (1, 2, 3, 40), the largest Matriplex width
-(bsize 1024), adjoints whose last checkpoint segment is shorter than the stride, and empty batches
-through every entry point.  Exact mode is compared word for word with the oracle."""
+"""GPU edge cases of the C-ABI: layer counts other than 20 (1, 2, 3, 40), batches ending inside a
+wavefront, the largest Matriplex width (bsize 1024), adjoints whose last checkpoint segment is shorter
+than the stride, and empty batches through every entry point.  Exact mode is compared word for word with the oracle."""
 import numpy as np
 import pytest
 
@@ -52,6 +52,20 @@ def test_exact_layer_counts(orc, ctx, semantics, nlayer):
     cpu, c2c = orc.run_f32(trk, hits, shape.nblk, nlayer, 32, semantics=0 if semantics == "reference" else 1,
                            chi2=True, cr_trig=True)
     assert _same(gpu, cpu) and _same(c2g, c2c)
+
+
+@pytest.mark.parametrize("nevts,ntrks,bsize", [(1, 96, 32), (3, 40, 8), (1, 1, 1), (3, 1120, 32)])
+def test_exact_partial_wavefronts(orc, ctx, nevts, ntrks, bsize):
+    """Batches that end inside a wavefront (96, 120, 1 and 3360 tracks): the last wavefront's active lanes
+    are a prefix, and its hit rows end at the batch's last block."""
+    import p2r_amd
+    shape = Shape(nevts, ntrks, 20, bsize)
+    trk, hits = p2r_amd.gen_uniform(shape)
+    for sem in ("reference", "chained"):
+        gpu, c2g = _forward(ctx, trk, hits, shape, sem)
+        cpu, c2c = orc.run_f32(trk, hits, shape.nblk, 20, bsize, semantics=0 if sem == "reference" else 1,
+                               chi2=True, cr_trig=True)
+        assert _same(gpu, cpu) and _same(c2g, c2c), sem
 
 
 def test_exact_widest_matriplex(orc, ctx):
