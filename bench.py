@@ -173,6 +173,13 @@ def main():
             cpu1 = orc.run_f32(t1, h1, nb1, shape.nlayer, shape.bsize, semantics=sem, cr_trig=True)
             same = (gpu1.view(np.uint32) == cpu1.view(np.uint32)) | (np.isnan(gpu1) & np.isnan(cpu1))
             parity_res = {"event0_bit_identical_to_oracle": bool(same.all()), "words_differing": int((~same).sum())}
+            # error of each quantity against the FP64 replay of the same algorithm, beside that of the
+            # restatement built with the reference's CPU flags (-O3 -ffast-math)
+            alt1 = oracle.Oracle("liboracle_bench.so").run_f32(t1, h1, nb1, shape.nlayer, shape.bsize, semantics=sem)
+            p64, c64, _ = orc.run_f64(t1, h1, nb1, shape.nlayer, shape.bsize, semantics=sem)
+            parity_res["event0_rel_err_vs_fp64"] = parity.errors_vs_fp64(gpu1, shape.bsize, p64, c64)
+            parity_res["event0_rel_err_vs_fp64_reference_cpu_flags"] = parity.errors_vs_fp64(alt1, shape.bsize, p64,
+                                                                                              c64)
         else:
             cpu1 = orc.run_f32(t1, h1, nb1, shape.nlayer, shape.bsize, semantics=sem)
             alt1 = oracle.Oracle("liboracle_bench.so").run_f32(t1, h1, nb1, shape.nlayer, shape.bsize, semantics=sem)

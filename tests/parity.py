@@ -119,6 +119,26 @@ def compare(gpu_buf, cpu_buf, bs, par64=None, cov64=None, alt_buf=None, valid=No
     return res
 
 
+def errors_vs_fp64(buf, bs, par64, cov64):
+    """Per-quantity error of an FP32 result against the FP64 replay, relative to each element's natural
+    scale (as above): max, 99th percentile and RMS over the finite tracks for x, y, z, ipt, phi, theta,
+    and over all 21 covariance entries together."""
+    p, c, _ = _arrays(buf, bs)
+    g = np.hstack([p, c])
+    f = np.hstack([par64, cov64])
+    sp, sc = scales(par64, cov64)
+    s = np.hstack([sp, sc])
+    ok = np.isfinite(g).all(axis=1) & np.isfinite(f).all(axis=1) & np.isfinite(s).all(axis=1)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        e = np.abs(g[ok] - f[ok]) / s[ok]
+    out = {"tracks": int(ok.sum())}
+    stat = lambda v: {"max": float(v.max()), "p99": float(np.quantile(v, 0.99)), "rms": float(np.sqrt(np.mean(v ** 2)))}
+    for k, name in enumerate(FIELD_NAMES[:6]):
+        out[name] = stat(e[:, k])
+    out["cov"] = stat(e[:, 6:].ravel())
+    return out
+
+
 def compare_statistical(gpu_buf, cpu_buf, alt_buf, bs, par64, cov64):
     """Criterion for an FP32 result computed with a different rounding (fused multiply-adds, hardware
     reciprocal).  The yardstick is the reference's own CPU build a (its build.py flags), which differs from

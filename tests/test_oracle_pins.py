@@ -365,3 +365,27 @@ def test_reference_driver_harness(orc, ref):
     f, w = fast.reshape(-1, 28, 32), want.reshape(-1, 28, 32)
     assert np.array_equal(f[:, 27].view(np.int32), w[:, 27].view(np.int32))
     assert np.isfinite(f[:, :27]).all()
+
+
+def test_errors_vs_fp64_measures_planted_perturbations(orc):
+    """The per-quantity error summary (reported by bench.py) on the oracle's own FP64 replay: rounding that
+    replay to FP32 gives errors at the FP32 rounding level, and a relative perturbation planted in one
+    parameter of every track reads back as that perturbation, in that parameter only."""
+    import parity
+    import p2r_amd
+    from p2r_amd import Shape
+    sh = Shape(1, 256, 20, BS)
+    trk, hits = p2r_amd.gen_uniform(sh)
+    p64, c64, _ = orc.run_f64(trk, hits, sh.nblk, 20, BS, semantics=1)
+    buf = np.zeros((sh.nblk, 28, BS), np.float32)
+    buf[:, :6, :] = p64.reshape(sh.nblk, BS, 6).transpose(0, 2, 1)
+    buf[:, 6:27, :] = c64.reshape(sh.nblk, BS, 21).transpose(0, 2, 1)
+    e = parity.errors_vs_fp64(buf.ravel(), BS, p64, c64)
+    assert e["tracks"] == 256
+    for name in ("x", "y", "z", "ipt", "phi", "theta"):
+        assert e[name]["max"] <= 2.0 ** -24 * 1.01, (name, e[name])
+    assert e["cov"]["max"] <= 2.0 ** -24 * 1.01
+    buf[:, 3, :] *= np.float32(1 + 1e-3)  # ipt
+    e2 = parity.errors_vs_fp64(buf.ravel(), BS, p64, c64)
+    assert abs(e2["ipt"]["max"] - 1e-3) < 2e-6 and abs(e2["ipt"]["rms"] - 1e-3) < 2e-6
+    assert e2["x"] == e["x"] and e2["cov"] == e["cov"]
