@@ -34,6 +34,67 @@ def _stream(stream):
     return C.c_void_p(s.cuda_stream)
 
 
+# ---- buffer checks --------------------------------------------------------------------------------------
+# The C-ABI takes raw pointers, so a buffer shorter than its shape implies, or a host array where a device
+# one is expected, would reach a kernel.  The methods below check what they pass: device buffers must be
+# contiguous CUDA tensors on the context's device, of the right dtype, holding at least the elements the
+# shape implies.  Sizes are checked only for shapes the library accepts (the same rules as include/p2r.h);
+# for the others the library's own status code is reported.
+_TORCH_DTYPES = {"f32": "float32", "i32": "int32", "f64": "float64"}
+
+
+def _check_dev(t, n, kind, device, name):
+    if t is None or n <= 0:
+        return
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name}: expected a CUDA tensor, got {type(t).__name__}"
+                        + ("" if not isinstance(t, torch.Tensor) else f" on {t.device}"))
+    if t.device.index != device:
+        raise ValueError(f"{name}: tensor on {t.device}, context on cuda:{device}")
+    if t.dtype != getattr(torch, _TORCH_DTYPES[kind]):
+        raise TypeError(f"{name}: dtype {t.dtype}, expected torch.{_TORCH_DTYPES[kind]}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: tensor is not contiguous")
+    if t.numel() < n:
+        raise ValueError(f"{name}: {t.numel()} elements, the shape needs {n}")
+
+
+def _check_host(t, n, name):
+    if t is None or n <= 0:
+        return
+    import torch
+    if isinstance(t, torch.Tensor):
+        ok = (not t.is_cuda) and t.dtype == torch.float32 and t.is_contiguous()
+        size = t.numel()
+    elif isinstance(t, np.ndarray):
+        ok = t.dtype == np.float32 and t.flags.c_contiguous
+        size = t.size
+    else:
+        raise TypeError(f"{name}: expected a host tensor or array, got {type(t).__name__}")
+    if not ok:
+        raise TypeError(f"{name}: expected contiguous float32 host memory")
+    if size < n:
+        raise ValueError(f"{name}: {size} elements, the shape needs {n}")
+
+
+def _uniform_ok(sh, max_layers=1 << 30):
+    """p2r_forward / p2r_forward_adjoint accept this shape (their host checks)."""
+    return (sh.nevts >= 0 and sh.ntrks >= 0 and 1 <= sh.bsize <= 1024 and 1 <= sh.nlayer <= max_layers
+            and sh.ntrks % sh.bsize == 0 and sh.nevts * sh.ntrks < (1 << 31))
+
+
+def _params_ok(p):
+    """The entry points accept these parameters (their host checks)."""
+    return p.b_scale != 0 and p.precision in (P2R_EXACT, P2R_FAST) and not (p.options & ~7)
+
+
+def _ragged_ok(offs, nlayer, bsize):
+    """p2r_forward_ragged accepts these offsets and sizes (its host checks)."""
+    return (len(offs) >= 1 and 1 <= nlayer <= 32 and 1 <= bsize <= 1024 and offs[0] == 0
+            and bool(np.all(np.diff(offs) >= 0)) and int(offs[-1]) * bsize < (1 << 31))
+
+
 PRECISION = {"exact": P2R_EXACT, "fast": P2R_FAST}
 GRAD_PRECISION = {"f32": P2R_GRAD_F32, "mixed": P2R_GRAD_MIXED, "f64": P2R_GRAD_F64}
 
@@ -94,12 +155,42 @@ class Context:
         if st != P2R_OK:
             raise P2RError(st, f"{what}: {lib().p2r_last_error(self._h).decode()}")
 
+    def _check_forward(self, trk, hits, out, sh, p, chi2, what):
+        if not (_uniform_ok(sh) and _params_ok(p)):
+            return
+        n = sh.nevts * sh.ntrks
+        for t, m, name in ((trk, 28 * n, "trk"), (hits, 9 * n * sh.nlayer, "hits"), (out, 28 * n, "out"),
+                           (chi2, n * (sh.nlayer if p.options & P2R_OPT_CHI2_PER_LAYER else 1), "chi2")):
+            _check_dev(t, m, "f32", self.device, f"{what}: {name}")
+
+    def _check_ragged(self, trk, hits, out, offs, mask, nlayer, bsize, p, chi2, what):
+        if not (_ragged_ok(offs, nlayer, bsize) and _params_ok(p)):
+            return
+        nslot = int(offs[-1]) * bsize
+        for t, m, name in ((trk, 28 * nslot, "trk"), (hits, 9 * nslot * nlayer, "hits"), (out, 28 * nslot, "out"),
+                           (chi2, nslot * (nlayer if p.options & P2R_OPT_CHI2_PER_LAYER else 1), "chi2")):
+            _check_dev(t, m, "f32", self.device, f"{what}: {name}")
+        _check_dev(mask, nslot, "i32", self.device, f"{what}: layer_mask")
+
+    def _check_adjoint(self, trk, hits, out, sh, p, gp, cs, chi2, grad_par, grad_hits, grad_kfact, grad_global,
+                       what):
+        if not (_uniform_ok(sh, 64) and _params_ok(p) and p.options == 0 and gp in (0, 1, 2)
+                and cs in (0, 1, 2, 4, 8)):
+            return
+        n, L = sh.nevts * sh.ntrks, sh.nlayer
+        for t, m, name in ((trk, 28 * n, "trk"), (hits, 9 * n * L, "hits"), (out, 28 * n, "out"), (chi2, n, "chi2"),
+                           (grad_par, 6 * n, "grad_par"), (grad_hits, 3 * n * L, "grad_hits"),
+                           (grad_kfact, n, "grad_kfact")):
+            _check_dev(t, m, "f32", self.device, f"{what}: {name}")
+        _check_dev(grad_global, 4 * L + 1, "f64", self.device, f"{what}: grad_global")
+
     def forward(self, trk, hits, out, shape, semantics="reference", params=None, chi2=None, stream=None,
                 precision=None):
         """Whole batch through all layers (reference layout; torch CUDA float32 tensors)."""
         sem = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
         sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
         p = params if params is not None else default_params(precision or "exact")
+        self._check_forward(trk, hits, out, sh, p, chi2, "p2r_forward")
         st = lib().p2r_forward(self._h, _ptr(trk), _ptr(hits), _ptr(out), C.byref(sh), sem, C.byref(p), _ptr(chi2),
                                _stream(stream))
         self._check(st, "p2r_forward")
@@ -111,6 +202,12 @@ class Context:
         sem = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
         sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
         p = params if params is not None else default_params(precision or "exact")
+        if _uniform_ok(sh) and 1 <= nstreams <= 16:
+            n = sh.nevts * sh.ntrks
+            for t, m, name in ((host_trk, 28 * n, "host_trk"), (host_hits, 9 * n * sh.nlayer, "host_hits"),
+                               (host_out, 28 * n, "host_out")):
+                _check_host(t, m, f"p2r_forward_streamed: {name}")
+            self._check_forward(dev_trk, dev_hits, dev_out, sh, p, None, "p2r_forward_streamed")
         st = lib().p2r_forward_streamed(self._h, _ptr(host_trk), _ptr(host_hits), _ptr(host_out), _ptr(dev_trk),
                                         _ptr(dev_hits), _ptr(dev_out), C.byref(sh), sem, C.byref(p), int(nstreams))
         self._check(st, "p2r_forward_streamed")
@@ -122,6 +219,7 @@ class Context:
         sem = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
         p = params if params is not None else default_params(precision or "exact")
         offs = np.ascontiguousarray(blk_offsets, np.int64)
+        self._check_ragged(trk, hits, out, offs, layer_mask, int(nlayer), int(bsize), p, chi2, "p2r_forward_ragged")
         st = lib().p2r_forward_ragged(self._h, _ptr(trk), _ptr(hits), _ptr(out), _ptr(offs), len(offs) - 1,
                                       _ptr(layer_mask), int(nlayer), int(bsize), sem, C.byref(p), _ptr(chi2),
                                       int(tile_rounds), _stream(stream))
@@ -136,6 +234,8 @@ class Context:
         sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
         p = params if params is not None else default_params(precision or "exact")
         gp = GRAD_PRECISION[grad_precision] if isinstance(grad_precision, str) else int(grad_precision)
+        self._check_adjoint(trk, hits, out, sh, p, gp, int(checkpoint_stride), chi2, grad_par, grad_hits, grad_kfact,
+                            grad_global, "p2r_forward_adjoint")
         st = lib().p2r_forward_adjoint(self._h, _ptr(trk), _ptr(hits), _ptr(out), C.byref(sh), C.byref(p), gp,
                                        int(checkpoint_stride), _ptr(chi2), _ptr(grad_par), _ptr(grad_hits),
                                        _ptr(grad_kfact), _ptr(grad_global), _stream(stream))
@@ -147,6 +247,7 @@ class Context:
         sem = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
         sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
         p = params if params is not None else default_params(precision or "exact")
+        self._check_forward(trk, hits, out, sh, p, chi2, "p2r_graph_forward")
         h = C.c_void_p()
         self._check(lib().p2r_graph_forward(self._h, _ptr(trk), _ptr(hits), _ptr(out), C.byref(sh), sem, C.byref(p),
                                             _ptr(chi2), C.byref(h)), "p2r_graph_forward")
@@ -157,6 +258,8 @@ class Context:
         sem = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
         p = params if params is not None else default_params(precision or "exact")
         offs = np.ascontiguousarray(blk_offsets, np.int64)
+        self._check_ragged(trk, hits, out, offs, layer_mask, int(nlayer), int(bsize), p, chi2,
+                           "p2r_graph_forward_ragged")
         h = C.c_void_p()
         self._check(lib().p2r_graph_forward_ragged(self._h, _ptr(trk), _ptr(hits), _ptr(out), _ptr(offs),
                                                    len(offs) - 1, _ptr(layer_mask), int(nlayer), int(bsize), sem,
@@ -170,6 +273,8 @@ class Context:
         sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
         p = params if params is not None else default_params(precision or "exact")
         gp = GRAD_PRECISION[grad_precision] if isinstance(grad_precision, str) else int(grad_precision)
+        self._check_adjoint(trk, hits, out, sh, p, gp, int(checkpoint_stride), chi2, grad_par, grad_hits, grad_kfact,
+                            grad_global, "p2r_graph_forward_adjoint")
         h = C.c_void_p()
         self._check(lib().p2r_graph_forward_adjoint(self._h, _ptr(trk), _ptr(hits), _ptr(out), C.byref(sh),
                                                     C.byref(p), gp, int(checkpoint_stride), _ptr(chi2),
@@ -196,6 +301,7 @@ class Context:
         sem = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
         sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
         p = params if params is not None else default_params(precision or "exact")
+        self._check_forward(trk, hits, out, sh, p, None, "p2r_autotune_forward")
         r = p2r_tune_report()
         self._check(lib().p2r_autotune_forward(self._h, _ptr(trk), _ptr(hits), _ptr(out), C.byref(sh), sem,
                                                C.byref(p), int(reps), C.byref(r), _stream(stream)),
@@ -207,6 +313,7 @@ class Context:
         sem = SEMANTICS[semantics] if isinstance(semantics, str) else int(semantics)
         p = params if params is not None else default_params(precision or "exact")
         offs = np.ascontiguousarray(blk_offsets, np.int64)
+        self._check_ragged(trk, hits, out, offs, layer_mask, int(nlayer), int(bsize), p, None, "p2r_autotune_ragged")
         r = p2r_tune_report()
         self._check(lib().p2r_autotune_ragged(self._h, _ptr(trk), _ptr(hits), _ptr(out), _ptr(offs), len(offs) - 1,
                                               _ptr(layer_mask), int(nlayer), int(bsize), sem, C.byref(p), int(reps),
@@ -218,6 +325,8 @@ class Context:
         sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
         p = params if params is not None else default_params(precision or "exact")
         gp = GRAD_PRECISION[grad_precision] if isinstance(grad_precision, str) else int(grad_precision)
+        self._check_adjoint(trk, hits, out, sh, p, gp, 0, chi2, grad_par, grad_hits, grad_kfact,
+                            grad_global, "p2r_autotune_adjoint")
         r = p2r_tune_report()
         self._check(lib().p2r_autotune_adjoint(self._h, _ptr(trk), _ptr(hits), _ptr(out), C.byref(sh), C.byref(p), gp,
                                                _ptr(chi2), _ptr(grad_par), _ptr(grad_hits), _ptr(grad_kfact),
@@ -227,6 +336,10 @@ class Context:
 
     def repack(self, src, src_bsize, dst, dst_bsize, nevts, ntrks, nsub, words, stream=None):
         """Matriplex width conversion on the device (MPTRK: nsub=1, words=28; MPHIT: nsub=nlayer, words=9)."""
+        total = int(nevts) * int(ntrks) * int(nsub) * int(words)
+        if total > 0 and nsub >= 1 and words >= 1:
+            _check_dev(src, total, "f32", self.device, "p2r_repack: src")
+            _check_dev(dst, total, "f32", self.device, "p2r_repack: dst")
         self._check(lib().p2r_repack(self._h, _ptr(src), int(src_bsize), _ptr(dst), int(dst_bsize), int(nevts),
                                      int(ntrks), int(nsub), int(words), _stream(stream)), "p2r_repack")
 
@@ -236,6 +349,7 @@ class Context:
         sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
         p = params if params is not None else default_params(precision or "exact")
         w = np.ascontiguousarray(widths, np.int32)
+        self._check_forward(trk, hits, None, sh, p, None, "p2r_autotune_width")
         r = p2r_tune_report()
         self._check(lib().p2r_autotune_width(self._h, _ptr(trk), _ptr(hits), C.byref(sh), sem, C.byref(p), _ptr(w),
                                              len(w), int(reps), C.byref(r), _stream(stream)), "p2r_autotune_width")

@@ -155,3 +155,29 @@ def test_torch_extension_loads_and_exports():
         assert hasattr(ag._ext, name)
     import p2r_amd
     assert ag._ext.default_kfact() == p2r_amd.default_params().kfact
+
+
+def test_python_buffer_checks_reject_host_memory_and_short_buffers():
+    """The Python mirror refuses what would reach a kernel as a bad pointer: host memory where device memory is
+    expected, and the shape rules it mirrors from the C-ABI decide when sizes are checked at all."""
+    torch = pytest.importorskip("torch")
+    import numpy as np
+    import p2r_amd
+    from p2r_amd import Shape, _check_dev, _check_host, _ragged_ok, _uniform_ok
+    with pytest.raises(TypeError):
+        _check_dev(torch.zeros(10), 5, "f32", 0, "trk")            # a CPU tensor as a device buffer
+    with pytest.raises(TypeError):
+        _check_dev(np.zeros(10, np.float32), 5, "f32", 0, "trk")   # a numpy array as a device buffer
+    _check_dev(None, 5, "f32", 0, "chi2")                           # optional buffers may be absent
+    _check_host(np.zeros(10, np.float32), 10, "host_trk")
+    with pytest.raises(ValueError):
+        _check_host(np.zeros(9, np.float32), 10, "host_trk")
+    with pytest.raises(TypeError):
+        _check_host(np.zeros(10, np.float64), 10, "host_trk")
+    ok = lambda *a: _uniform_ok(p2r_amd.p2r_shape(*a))
+    assert ok(1, 64, 20, 32) and ok(0, 32, 20, 32)
+    assert not ok(1, 65, 20, 32) and not ok(1, 64, 0, 32) and not ok(1, 64, 20, 2048)
+    assert not ok(70000, 65536, 20, 32) and not _uniform_ok(p2r_amd.p2r_shape(1, 64, 65, 32), 64)
+    offs = np.array([0, 1, 3], np.int64)
+    assert _ragged_ok(offs, 20, 32) and not _ragged_ok(offs, 33, 32) and not _ragged_ok(np.array([1, 2]), 20, 32)
+    assert not _ragged_ok(np.array([0, 2, 1]), 20, 32)

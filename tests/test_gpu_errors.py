@@ -123,3 +123,41 @@ def test_tuning_streamed_and_repack_errors(ctx):
     b = torch.zeros(28 * 64, device="cuda")
     assert _status(lambda: ctx.repack(a, 48, b, 16, 1, 64, 1, 28)) == p2r_amd.P2R_ERR_SHAPE  # 48 does not divide 64
     ctx.repack(a, 32, b, 16, 1, 64, 1, 28)  # the context still works
+
+
+def test_python_mirror_checks_buffers_before_launch(ctx):
+    """Buffers shorter than the shape implies, of the wrong dtype, not contiguous, or in host memory are
+    refused by the Python mirror before anything reaches the device; the context stays usable."""
+    import p2r_amd
+    s = Shape(1, 64, 20, 32)
+    t = torch.zeros(28 * 64, device="cuda")
+    h = torch.zeros(9 * 64 * 20, device="cuda")
+    o = torch.zeros(28 * 64, device="cuda")
+    with pytest.raises(ValueError):
+        ctx.forward(t[:-1], h, o, s)
+    with pytest.raises(ValueError):
+        ctx.forward(t, h[: 9 * 64 * 19], o, s)
+    with pytest.raises(ValueError):
+        ctx.forward(t, h, o, s, chi2=torch.zeros(63, device="cuda"))
+    with pytest.raises(ValueError):  # per-layer chi2 needs nlayer x n
+        ctx.forward(t, h, o, s, chi2=torch.zeros(64, device="cuda"),
+                    params=p2r_amd.default_params(options=p2r_amd.P2R_OPT_CHI2_PER_LAYER))
+    with pytest.raises(TypeError):
+        ctx.forward(t.double(), h, o, s)
+    with pytest.raises(TypeError):
+        ctx.forward(t.cpu(), h, o, s)
+    with pytest.raises(ValueError):
+        ctx.forward(torch.zeros(2 * 28 * 64, device="cuda")[::2], h, o, s)
+    offs = np.array([0, 2], np.int64)
+    m = torch.zeros(64, dtype=torch.int32, device="cuda")
+    with pytest.raises(ValueError):
+        ctx.forward_ragged(t, h, o, offs, m[:63], 20, 32)
+    with pytest.raises(TypeError):
+        ctx.forward_ragged(t, h, o, offs, m.float(), 20, 32)
+    with pytest.raises(ValueError):
+        ctx.forward_adjoint(t, h, o, s, grad_hits=torch.zeros(3 * 64 * 19, device="cuda"))
+    with pytest.raises(TypeError):
+        ctx.forward_adjoint(t, h, o, s, grad_global=torch.zeros(81, device="cuda"))
+    ctx.forward(t, h, o, s)  # still usable
+    torch.cuda.synchronize()
+
