@@ -52,10 +52,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="time only the headline variant")
-    ap.add_argument("--no-ragged", action="store_true", help="skip the uneven-batch (config 3) measurement")
+    ap.add_argument("--no-ragged", action="store_true", help="skip the uneven-batch measurement (500 events, masked layers)")
     ap.add_argument("--ragged-events", type=int, default=500)
-    ap.add_argument("--no-adjoint", action="store_true", help="skip the fwd+adjoint (config 4) measurement")
-    ap.add_argument("--no-alignment", action="store_true", help="skip the alignment (config 5) measurement")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the fwd+adjoint measurement (100 x 9600 x 20)")
+    ap.add_argument("--no-alignment", action="store_true", help="skip the alignment measurement (1e6 tracks x 40 layers)")
     ap.add_argument("--alignment-tracks", type=int, default=1000000)
     ap.add_argument("--no-parity", action="store_true", help="skip the parity spot check")
     ap.add_argument("--no-autotune", action="store_true", help="time the library's default launch choices")
@@ -250,7 +250,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded counter-based generator, BASELINE config 2 shape)",
+        "data": "synthetic (seeded counter-based generator, 100 events x 9600 tracks x 20 layers)",
         "config": {"workload": f"p2r fwd, {args.nevts} events x {args.ntrks} tracks x {args.nlayer} layers, "
                                f"bsize {args.bsize}, {args.semantics} semantics, {args.precision} FP32",
                    "nevts": args.nevts, "ntrks": args.ntrks, "nlayer": args.nlayer, "bsize": args.bsize,
@@ -291,7 +291,7 @@ def _timed(fn, reps=5, warm=2):
 
 
 def adjoint_bench(ctx, args, variant_ms, headline):
-    """the headline shape, chained fit + reverse-mode gradient of the total chi2 with
+    """Fit + adjoint: the headline shape (100 x 9600 x 20), chained fit + reverse-mode gradient of the total chi2 with
     respect to the initial parameters, every hit coordinate and kfact (checkpoints every 4 layers).
     Reported as the time of one fused forward+adjoint launch and its ratio to the plain chained forward."""
     import torch
@@ -322,7 +322,7 @@ def adjoint_bench(ctx, args, variant_ms, headline):
 
 
 def alignment_bench(ctx, args):
-    """1e6 tracks x 40 layers, hits generated with hidden per-layer rigid offsets
+    """Alignment workload: 1e6 tracks x 40 layers, hits generated with hidden per-layer rigid offsets
     (dx, dy, dz ~ N(0, 50 um), dphi ~ N(0, 1e-4)) and field scale 1 + N(0, 1e-3); the fit runs at nominal
     geometry and field; fwd+adjoint gives d(sum chi2)/d(160 offsets + field scale), reduced in FP64.
     Reports the time and the relative error of the FP32 / mixed gradients against the FP64 one."""
@@ -368,7 +368,7 @@ def alignment_bench(ctx, args):
 
 
 def ragged_bench(ctx, args, uniform_chained, headline):
-    """events with lognormal track counts (median 9600, sigma 0.5, clipped to
+    """Uneven batch: 500 events with lognormal track counts (median 9600, sigma 0.5, clipped to
     [500, 40000]), hit count per track uniform in 5..20 (random subset of the 20 layers), chained
     semantics, one launch replayed from a captured HIP graph.  For each Matriplex width 16/32/64 (inputs
     generated in that width) the autotuner times the schedules (layer order at 64/128/256 lanes per
@@ -439,7 +439,7 @@ def ragged_bench(ctx, args, uniform_chained, headline):
 
 def measured_traffic():
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
-    p = os.path.join(ROOT, "profiles", "traffic_fwd_uniform_config2.json")
+    p = os.path.join(ROOT, "profiles", "traffic_fwd_uniform_100x9600x20.json")
     if not os.path.exists(p):
         return None
     try:

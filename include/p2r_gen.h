@@ -3,7 +3,7 @@
  *
  * Buffers are host memory in the Matriplex layout of p2r.h.  These replace the
  * reference's prepareTracks / prepareHits (propagate-tor-test_tbb.cpp:243-293)
- * and add the seeded generator the BASELINE configurations are defined on.
+ * and add the seeded generator of the benchmark workloads.
  */
 #ifndef P2R_GEN_H
 #define P2R_GEN_H
@@ -27,7 +27,7 @@ int p2r_gen_reference(float* trk, float* hits, const p2r_shape* shape);
  * reference summary compares against, tbb.cpp:948-951). */
 int p2r_reference_hit(int layer, float pos[3]);
 
-/* Seeded counter-based generator for the BASELINE configurations: the base
+/* Seeded counter-based generator for the benchmark workloads: the base
  * state of propagate-tor-test_acc.v3.cpp:808-812 smeared by (1 + smear*N(0,1)),
  * q = +-1 with equal probability, one hit per layer l on the cylinder
  * r_l = r0 + l*dr, hit cov diag(hit_var) (entries xx, yy, zz).  Every value is

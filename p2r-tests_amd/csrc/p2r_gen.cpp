@@ -71,7 +71,7 @@ inline double normal_trunc3(uint64_t seed, uint64_t stream, uint64_t item, uint6
 }
 inline uint64_t track_item(uint64_t ev, uint64_t tk) { return (ev << 32) | tk; }
 
-// Base state of the BASELINE configurations (propagate-tor-test_acc.v3.cpp:808-812).
+// Base state of the benchmark workloads (propagate-tor-test_acc.v3.cpp:808-812).
 const double kBasePar[6] = {-12.806846618652344, -7.723824977874756, 38.13014221191406,
                             0.23732035065189902, -2.613372802734375, 0.35594117641448975};
 const double kBaseCov[21] = {6.290299552347278e-07, 4.1375109560704004e-08, 7.526661534029699e-07,

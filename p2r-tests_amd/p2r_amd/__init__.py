@@ -403,7 +403,7 @@ def gen_config(**kw):
 
 
 def gen_uniform(shape, cfg=None):
-    """Seeded BASELINE-style inputs (include/p2r_gen.h) as host float32 arrays."""
+    """Seeded benchmark inputs (include/p2r_gen.h) as host float32 arrays."""
     cfg = cfg if cfg is not None else gen_config()
     trk = np.zeros(shape.trk_floats, np.float32)
     hits = np.zeros(shape.hit_floats, np.float32)

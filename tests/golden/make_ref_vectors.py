@@ -34,7 +34,7 @@ BS = 32
 
 
 def wide_block(seed, nlayer):
-    """One Matriplex block of BASELINE-style inputs (numpy PCG64, stable)."""
+    """One Matriplex block of benchmark-style inputs (numpy PCG64, stable)."""
     rng = np.random.default_rng(seed)
     base = np.array([-12.806846618652344, -7.723824977874756, 38.13014221191406, 0.23732035065189902,
                      -2.613372802734375, 0.35594117641448975])

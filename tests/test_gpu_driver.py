@@ -70,7 +70,7 @@ def test_driver_c0_protocol_and_summary(c0_expected, graph):
 
 
 def test_driver_baseline_inputs_and_transfers():
-    """BASELINE inputs, every timed-region choice of the reference (transfers none / h2d / d2h / both with
+    """Seeded benchmark inputs, every timed-region choice of the reference (transfers none / h2d / d2h / both with
     streams): each run prints a done line and the same summary (the output does not depend on the timing)."""
     outs = [_run(["--input", "baseline", "--nevts", "4", "--NITER", "1"] + extra)
             for extra in ([], ["--transfers", "h2d"], ["--transfers", "d2h"],

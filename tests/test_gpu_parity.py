@@ -107,12 +107,12 @@ def test_exact_reference_inputs(orc, ctx, semantics):
 
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("semantics", ["reference", "chained"])
-def test_exact_baseline_config1(orc, ctx, semantics, mode):
-    """BASELINE config 1 (10 events x 9600 tracks x 20 layers), consistent (0) and literal (1) inputs."""
+def test_exact_10x9600x20(orc, ctx, semantics, mode):
+    """10 events x 9600 tracks x 20 layers of seeded benchmark inputs, consistent (0) and literal (1) inputs."""
     import p2r_amd
     shape = Shape(10, 9600, 20, 32)
     trk, hits = p2r_amd.gen_uniform(shape, p2r_amd.gen_config(mode=mode))
-    check_exact(orc, ctx, trk, hits, shape, semantics, f"config1-mode{mode}")
+    check_exact(orc, ctx, trk, hits, shape, semantics, f"10x9600x20-mode{mode}")
 
 
 @pytest.mark.parametrize("bsize", [1, 16, 64, 128])
@@ -199,13 +199,13 @@ def test_fast_reference_inputs(orc, orc_alt, ctx, semantics):
 
 
 @pytest.mark.parametrize("semantics,mode", [("reference", 0), ("chained", 0), ("reference", 1)])
-def test_fast_baseline_config1(orc, orc_alt, ctx, semantics, mode):
+def test_fast_10x9600x20(orc, orc_alt, ctx, semantics, mode):
     """Consistent inputs in both semantics; literal inputs in reference semantics only (literal chained
     fits break down in FP32 for ~90 % of tracks; they are covered bit-exactly by the exact tests)."""
     import p2r_amd
     shape = Shape(10, 9600, 20, 32)
     trk, hits = p2r_amd.gen_uniform(shape, p2r_amd.gen_config(mode=mode))
-    check_fast(orc, orc_alt, ctx, trk, hits, shape, semantics, f"config1-mode{mode}")
+    check_fast(orc, orc_alt, ctx, trk, hits, shape, semantics, f"10x9600x20-mode{mode}")
 
 
 @pytest.mark.parametrize("semantics", ["reference", "chained"])
