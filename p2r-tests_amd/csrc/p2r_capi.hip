@@ -104,46 +104,64 @@ struct WaveHits {
 // Workgroups of 64..256 lanes (blockDim.x).
 // MINW: minimum waves per SIMD the register allocation must allow (1 = the compiler's choice; 3 or 4
 // trade spills for occupancy, a tuning choice).
+// In reference semantics every layer restarts from the input track, whose covariance the propagation
+// reads only in the transport at its end: it waits in LDS (one column of [21][blockDim.x] per lane)
+// instead of holding 21 registers through the helix iterations.  The parameters stay in registers.
+struct LdsCov {
+  const float* lds;
+  uint32_t col, stride;
+  __device__ __forceinline__ float operator[](int i) const { return lds[(uint32_t)i * stride + col]; }
+};
+
 template <int SEM, bool CHI2, bool EXACT, int MINW>
 __global__ __launch_bounds__(kThreads, MINW) void fwd_uniform_kernel(const float* trk, const float* __restrict__ hits,
                                                                float* out, float* __restrict__ chi2, uint32_t ntrk,
                                                                uint32_t bs, int nlayer, float kfact, int l0, int l1,
                                                                bool acc, int opts) {
+  extern __shared__ float cov_lds[];  // [21][blockDim.x], reference semantics only
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ntrk) return;
-  const uint32_t blk = g / bs, lane = g - blk * bs;
+  const uint32_t blk = g / bs, lane = g - blk * bs, stride = blockDim.x;
   const float* tb = trk + (size_t)blk * 28 * bs + lane;
-  Trk in;
-  load_trk(tb, bs, in);
+  Trk cur;  // chained: the state between layers
+  float in_par[6];
+  if (SEM == P2R_REFERENCE) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) in_par[f] = tb[(size_t)f * bs];
+#pragma unroll
+    for (int f = 0; f < 21; ++f) cov_lds[f * stride + threadIdx.x] = tb[(size_t)(6 + f) * bs];
+  } else {
+    load_trk(tb, bs, cur);
+  }
   const int q_in = reinterpret_cast<const int*>(tb)[(size_t)27 * bs];
   int q = q_in;
   float k = (float)q * kfact;  // tbb.cpp:667
 
   const WaveHits wh(hits, g, ntrk, bs, nlayer, blk, lane);
-  Trk cur = in;
   const bool per_layer = opts & P2R_OPT_CHI2_PER_LAYER;
   // a later launch of a split layer loop continues the running sum (same rounding as one launch)
   float c2 = (CHI2 && acc && !per_layer) ? chi2[g] : 0.f;
   // Each layer loads its hit at the top of the iteration.  Prefetching the next layer's hit into
   // registers (9 values live across the whole update) took the kernel to 152-168 VGPRs, 3 waves per
-  // SIMD; without it the exact kernel needs 122 VGPRs (4 waves) and the fast one 95 (5 waves), and the
-  // extra waves hide the load latency: measured 5-7 % faster.
+  // SIMD; loading it as the layer starts leaves the extra waves to hide the load latency.
   for (int l = l0; l < l1; ++l) {
     Hit h;
     wh.load(l, h);
-    Trk src;
+    Trk prop;
     if (SEM == P2R_REFERENCE) {
-      // tbb.cpp:894: every layer starts from the input track.  The barrier, on the input itself (in place,
-      // no copy: its values never change), makes every layer recompute; nothing is hoisted across layers.
-      opaque(in);
-      src = in;
+      // tbb.cpp:894: every layer starts from the input track.  The barriers, on the input parameters in
+      // place and on the covariance column's offset, make every layer recompute and reread it; nothing is
+      // hoisted across layers.
+#pragma unroll
+      for (int f = 0; f < 6; ++f) opaque(in_par[f]);
+      uint32_t col = threadIdx.x;
+      asm volatile("" : "+v"(col));
       q = q_in;
       k = (float)q * kfact;
+      Lane<EXACT>::propagate(in_par, LdsCov{cov_lds, col, stride}, k, h.pos[0], h.pos[1], prop);
     } else {
-      src = cur;     // propagate-tor-test_hip.cu:842-845
+      Lane<EXACT>::propagate(cur, k, h.pos[0], h.pos[1], prop);  // propagate-tor-test_hip.cu:842-845
     }
-    Trk prop;
-    Lane<EXACT>::propagate(src, k, h.pos[0], h.pos[1], prop);
     float c = Lane<EXACT>::update(prop, h);
     if (opts) apply_options<EXACT>(opts, prop, q, k, kfact);
     if (SEM == P2R_REFERENCE) {
@@ -886,8 +904,9 @@ struct UniformLaunch {
 template <int SEM, bool CHI2, bool EXACT, int MINW>
 void launch_uniform_w(dim3 grid, dim3 block, hipStream_t st, const UniformLaunch& a, const float* in, int l0, int l1,
                       bool acc) {
-  fwd_uniform_kernel<SEM, CHI2, EXACT, MINW><<<grid, block, 0, st>>>(in, a.H, a.O, a.chi2, a.n, a.bs, a.nlayer, a.kf,
-                                                                     l0, l1, acc, a.opts);
+  fwd_uniform_kernel<SEM, CHI2, EXACT, MINW><<<grid, block, SEM == P2R_REFERENCE ? 21 * block.x * sizeof(float) : 0,
+                                               st>>>(
+      in, a.H, a.O, a.chi2, a.n, a.bs, a.nlayer, a.kf, l0, l1, acc, a.opts);
 }
 template <int SEM, bool CHI2, bool EXACT>
 void launch_uniform_e(int waves, dim3 grid, dim3 block, hipStream_t st, const UniformLaunch& a, const float* in,

@@ -119,12 +119,22 @@ template <> struct Lane<true> {
   static __device__ __forceinline__ void propagate(const Trk& in, float k, float hx, float hy, Trk& out) {
     exact::propagate_to_r(in, k, hx, hy, out);
   }
+  template <class PC>
+  static __device__ __forceinline__ void propagate(const float (&par)[6], const PC& P, float k, float hx, float hy,
+                                                   Trk& out) {
+    exact::propagate_to_r_c(par, P, k, hx, hy, out);
+  }
   static __device__ __forceinline__ float update(Trk& t, const Hit& h) { return exact::kalman_update(t, h); }
   static __device__ __forceinline__ float wrap_phi(float phi) { return exact::m_wrap_phi(phi); }
 };
 template <> struct Lane<false> {
   static __device__ __forceinline__ void propagate(const Trk& in, float k, float hx, float hy, Trk& out) {
     fast::propagate_to_r(in, k, hx, hy, out);
+  }
+  template <class PC>
+  static __device__ __forceinline__ void propagate(const float (&par)[6], const PC& P, float k, float hx, float hy,
+                                                   Trk& out) {
+    fast::propagate_to_r_c(par, P, k, hx, hy, out);
   }
   static __device__ __forceinline__ float update(Trk& t, const Hit& h) { return fast::kalman_update(t, h); }
   static __device__ __forceinline__ float wrap_phi(float phi) { return fast::m_wrap_phi(phi); }
