@@ -32,6 +32,11 @@ for _p in (ROOT, os.path.join(ROOT, "p2r-tests_amd"), os.path.join(ROOT, "tests"
 # the reference source, of which 63 multiply/add a structural 0 or 1 of the
 # Jacobian; the roofline credits the 1212 non-trivial ones.
 FLOP_PER_UPDATE = 1212
+# FLOPs of the reverse sweep of one track-layer (the adjoint of the update, its chi2 and the propagation),
+# counted on the product's own per-lane code by tests/opcount_harness.cpp: 2064 plus 246 for each fixed-point
+# iteration whose convergence branch is taken, 2408 on average over the benchmark's tracks.  The reverse's
+# recompute of the forward values (checkpointing) is implementation work and not credited.
+REVERSE_FLOP_PER_UPDATE = 2408
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (AMD published)
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E (AMD published)
 HIT_BYTES = 36             # per track-layer
@@ -180,6 +185,14 @@ def main():
             parity_res["event0_rel_err_vs_fp64"] = parity.errors_vs_fp64(gpu1, shape.bsize, p64, c64)
             parity_res["event0_rel_err_vs_fp64_reference_cpu_flags"] = parity.errors_vs_fp64(alt1, shape.bsize, p64,
                                                                                               c64)
+            # the 1e-4 bar against the reference's CPU path: the oracle with libm's sinf/cosf is word for word
+            # the reference's compiled routines, so this compares this run's GPU output with the reference;
+            # beside it the same numbers for the reference's algorithm built with its own flags
+            ref1 = orc.run_f32(t1, h1, nb1, shape.nlayer, shape.bsize, semantics=sem)
+            v = parity.vs_reference(gpu1, ref1, alt1, shape.bsize)
+            for d in (v, v["reference_build_flags"]):
+                d.pop("per_field_frac_within_1e-4", None)
+            parity_res["event0_vs_reference_cpu"] = v
         else:
             cpu1 = orc.run_f32(t1, h1, nb1, shape.nlayer, shape.bsize, semantics=sem)
             alt1 = oracle.Oracle("liboracle_bench.so").run_f32(t1, h1, nb1, shape.nlayer, shape.bsize, semantics=sem)
@@ -302,7 +315,8 @@ def adjoint_bench(ctx, args, variant_ms, headline):
     gpar = torch.empty(shape.nblk * 6 * shape.bsize, device="cuda")
     ghit = torch.empty(shape.nblk * shape.nlayer * 3 * shape.bsize, device="cuda")
     gk = torch.empty(n, device="cuda")
-    res = {"shape": list(shape.tuple()), "checkpoint_stride": 4, "stride_sweep": {}}
+    res = {"shape": list(shape.tuple()), "checkpoint_stride": 4, "stride_sweep": {},
+           "flop_per_update": FLOP_PER_UPDATE + REVERSE_FLOP_PER_UPDATE, "default_grad_precision": "mixed"}
     for prec in ("exact", "fast"):
         fwd_ms = variant_ms.get(("chained", prec))
         if fwd_ms is None and headline[0] == "chained" and headline[1] == prec:
@@ -312,8 +326,10 @@ def adjoint_bench(ctx, args, variant_ms, headline):
                 ms = _timed(lambda: ctx.forward_adjoint(d_trk, d_hit, d_out, shape, grad_precision=gp, precision=prec,
                                                         checkpoint_stride=cs, chi2=c2, grad_par=gpar, grad_hits=ghit,
                                                         grad_kfact=gk))
-                r = {"ms": ms, "updates_per_s": shape.ntrk_total * shape.nlayer / (ms * 1e-3),
-                     "ratio_to_chained_forward": (ms / fwd_ms) if fwd_ms else None}
+                rate = shape.ntrk_total * shape.nlayer / (ms * 1e-3)
+                r = {"ms": ms, "updates_per_s": rate,
+                     "ratio_to_chained_forward": (ms / fwd_ms) if fwd_ms else None,
+                     "roofline_frac": rate * (FLOP_PER_UPDATE + REVERSE_FLOP_PER_UPDATE) / 1e12 / PEAK_FP32_TFLOPS}
                 if cs == 4:
                     res[f"{prec}_{gp}"] = r
                 else:
@@ -433,7 +449,8 @@ def ragged_bench(ctx, args, uniform_chained, headline):
         if uni is None and headline[0] == "chained" and headline[1] == prec:
             uni = headline[2]
         res[prec] = {"best_bsize": bs, "best_schedule": sched, "best_threads": thr, "ms": ms, "updates_per_s": rate,
-                     "uniform_chained_updates_per_s": uni, "fraction_of_uniform": (rate / uni) if uni else None}
+                     "uniform_chained_updates_per_s": uni, "fraction_of_uniform": (rate / uni) if uni else None,
+                     "roofline_frac": rate * FLOP_PER_UPDATE / 1e12 / PEAK_FP32_TFLOPS}
     return res
 
 
@@ -448,17 +465,47 @@ def measured_traffic():
         return None
 
 
+def host_cpu():
+    """What this process may run on: logical CPUs of the host, those in its affinity mask, the thread count
+    the environment grants (OMP_NUM_THREADS, the host's share per GPU where one is set), and the CPU model
+    as lscpu names it."""
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    omp = os.environ.get("OMP_NUM_THREADS")
+    granted = int(omp) if omp and omp.isdigit() and int(omp) > 0 else affinity
+    model = ""
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for l in out.splitlines():
+            if l.strip().lower().startswith("model name"):
+                model = l.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    if not model:
+        try:
+            for l in open("/proc/cpuinfo"):
+                if l.startswith("model name"):
+                    model = l.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+    return {"os_cpu_count": os.cpu_count(), "affinity_cpus": affinity, "omp_num_threads": omp,
+            "threads_used": max(1, min(affinity, granted)), "model": model}
+
+
 def cpu_baseline(shape, trk, hits, semantics, seconds):
-    """The reference CPU path on this host's cores, on a bounded sample (leading events of the same inputs),
-    each leg repeated until `seconds` of wall time.  Kind "reference": the reference's own acc.v3 block
-    routines compiled with its CPU flags (-O3 -ffast-math) in tbb.cpp's driver loop, OpenMP over Matriplex
-    blocks (oracle/ref_acc_bench.cpp; reference semantics, bsize 32).  Kind "port": the oracle's
-    restatement with the same flags, used when that build is absent or for chained semantics; its rate is
-    reported beside the reference's either way."""
+    """The reference CPU path on every thread this process is granted, over the whole benchmark batch (all
+    events of the same inputs), each leg repeated until `seconds` of wall time.  Kind "reference": the
+    reference's own acc.v3 block routines compiled with its CPU flags (-O3 -ffast-math) in tbb.cpp's driver
+    loop, OpenMP over Matriplex blocks (oracle/ref_acc_bench.cpp; reference semantics, bsize 32).  Kind
+    "port": the oracle's restatement with the same flags, used when that build is absent or for chained
+    semantics; its rate is reported beside the reference's either way."""
     import oracle
     from p2r_amd import Shape
-    threads = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
-    nev = min(shape.nevts, 10)
+    host = host_cpu()
+    threads = host["threads_used"]
+    nev = shape.nevts
     s = Shape(nev, shape.ntrks, shape.nlayer, shape.bsize)
     t1, h1 = trk[: s.trk_floats], hits[: s.hit_floats]
 
@@ -482,20 +529,12 @@ def cpu_baseline(shape, trk, hits, semantics, seconds):
         if rb.bs == s.bsize:
             reps, el, rate = timed(lambda: rb.run(t1, h1, s.nblk, s.nlayer, threads))
             kind, how = "reference", "the reference's acc.v3.cpp block routines in tbb.cpp's driver loop"
-    model = ""
-    try:
-        for l in open("/proc/cpuinfo"):
-            if l.startswith("model name"):
-                model = l.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
     return {"value": rate, "unit": "track-layer updates/s", "cores": threads, "kind": kind,
-            "sample": f"{nev} events x {shape.ntrks} tracks x {shape.nlayer} layers (leading events of the "
-                      f"benchmark inputs), {reps} repetitions in {el:.1f} s: {how}, OpenMP over Matriplex "
-                      f"blocks, -O3 -ffast-math -march=x86-64-v3",
+            "sample": f"the whole batch, {nev} events x {shape.ntrks} tracks x {shape.nlayer} layers, "
+                      f"{reps} repetitions in {el:.1f} s: {how}, OpenMP over Matriplex blocks on {threads} "
+                      f"threads, -O3 -ffast-math -march=x86-64-v3",
             "port_value": prate, "port_sample": f"{preps} repetitions in {pel:.1f} s",
-            "cpu_model": model, "semantics": semantics}
+            "cpu_model": host["model"], "host": host, "semantics": semantics}
 
 
 if __name__ == "__main__":

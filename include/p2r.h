@@ -45,7 +45,10 @@ typedef enum p2r_semantics { P2R_REFERENCE = 0, P2R_CHAINED = 1 } p2r_semantics;
 typedef struct p2r_ctx p2r_ctx;
 
 /* Batch shape: the reference's compile-time nevts, ntrks, nlayer, bsize
- * (tbb.cpp:15-34), here runtime values.  bsize in [1, 1024]. */
+ * (tbb.cpp:15-34), here runtime values.  bsize in [1, 1024]; nlayer in
+ * [1, P2R_MAX_NLAYER] (at most 32 for p2r_forward_ragged, whose layer mask has
+ * 32 bits); other values return P2R_ERR_SHAPE. */
+enum { P2R_MAX_NLAYER = 64 };
 typedef struct p2r_shape {
   int32_t nevts;
   int32_t ntrks;
@@ -111,7 +114,8 @@ const char* p2r_last_error(const p2r_ctx* ctx);
  * (tbb.cpp:873-925 per repetition; GPU twins launch_p2r_kernel,
  * propagate-tor-test_hip_local.cu:911-939, and GPUsequence,
  * propagate-tor-test_hip.cu:820-849).  trk_in / trk_out: nevts*nb MPTRK
- * blocks; hits: nlayer*nevts*nb MPHIT blocks.  trk_out may equal trk_in.
+ * blocks; hits: nlayer*nevts*nb MPHIT blocks.  trk_out may equal trk_in
+ * (also when the context's tuning splits the layers over several launches).
  * chi2 (nullable): per-track sum over layers of r^T S^-1 r (the local chi2
  * the reference leaves commented out, tbb.cpp:631), float[nevts*ntrks] by flat
  * track index.  Output q equals input q (the reference copies it, tbb.cpp:880). */
@@ -147,7 +151,10 @@ int p2r_forward_ragged(p2r_ctx* ctx, const void* trk_in, const void* hits, void*
                        int32_t nevts, const uint32_t* layer_mask, int32_t nlayer, int32_t bsize, int semantics,
                        const p2r_params* params, float* chi2, int32_t tile_rounds, void* stream);
 
-/* Gradient precision of p2r_forward_adjoint.
+/* Gradient precision of p2r_forward_adjoint.  P2R_GRAD_MIXED is the one whose
+ * per-track gradients stay within 1e-3 of FP64 central differences (the
+ * Python mirror's default); all-FP32 gradients of ill-conditioned tracks can
+ * be off by ~1e-2, because the FP32 covariance update cancels.
  * The forward pass (trk_out, chi2) uses params->precision as p2r_forward does;
  * the reverse sweep uses fused multiply-adds whatever that precision.
  * P2R_GRAD_F32:   every layer in FP32.
@@ -164,9 +171,15 @@ typedef enum p2r_grad_precision { P2R_GRAD_F32 = 0, P2R_GRAD_MIXED = 1, P2R_GRAD
  * each segment from its checkpoint instead of storing every intermediate
  * (stride 1 stores every layer's input state and recomputes nothing but the
  * layer's own intermediates).  Workspace: (ceil(nlayer/stride) - 1) * 27 *
- * ntrk states of 4 bytes (P2R_GRAD_F32) or 8 bytes.
+ * ntrk states of 4 bytes (P2R_GRAD_F32) or 8 bytes.  The context holds one
+ * workspace: adjoint calls on one context must not overlap in time (issue
+ * them on one stream, or synchronize between streams or threads).
+ * trk_out must not overlap trk_in (the reverse sweep rereads the input
+ * tracks after the forward pass has written trk_out): P2R_ERR_ARG.
  * Outputs (device memory; any may be NULL except trk_out):
- *   trk_out     final fitted tracks (same as p2r_forward, P2R_CHAINED)
+ *   trk_out     final fitted tracks: with P2R_GRAD_F32 word for word those of
+ *               p2r_forward (P2R_CHAINED, same precision); with MIXED and F64
+ *               the fit carried in FP64, rounded to float
  *   chi2        float[ntrk]: total chi2 per track (flat track index)
  *   grad_par    float, MP6F blocks: d chi2 / d (initial x, y, z, 1/pt, phi, theta)
  *   grad_hits   float, one MP3F block per hit block (index block*nlayer + L):
@@ -221,11 +234,11 @@ int p2r_forward_streamed(p2r_ctx* ctx, const void* host_trk_in, const void* host
  * (pointers, shape, params, the context's tuning at capture time) into an
  * instantiated HIP graph, then replayed with p2r_graph_launch on any stream.
  * The enqueue paths issue kernel launches only (no allocation, no host
- * synchronisation, no memset nodes); the adjoint's workspace is sized before its
- * capture and must not be grown by a later call with a larger shape while the
- * graph is in use (the context keeps the buffer; a larger adjoint call on the same
- * context replaces it).  The ragged entry reads blk_offsets on the host at capture
- * time.  Replays run concurrently with nothing else on the context's workspace.
+ * synchronisation, no memset nodes).  An adjoint graph owns a workspace of its
+ * own, allocated before the capture and freed by p2r_graph_destroy, so later
+ * calls on the context leave it alone; replays of one adjoint graph must not
+ * overlap each other in time.  The ragged entry reads blk_offsets on the host
+ * at capture time.
  * Not in the reference (its drivers launch directly, hip.cu:963-975). */
 typedef struct p2r_graph p2r_graph;
 int p2r_graph_forward(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* shape,
@@ -278,6 +291,8 @@ typedef struct p2r_tune_report {
  * fastest in the context's tuning and fills `report` (nullable).  They
  * synchronize the stream and create events: not for use under graph capture.
  * Outputs are overwritten with valid results of the last candidate run.
+ * trk_out must not overlap trk_in (every candidate starts from the same
+ * input): P2R_ERR_ARG.
  *   forward: fwd_threads {64, 128, 256} x fwd_layers_per_launch {all,
  *            ceil(nlayer/2), ceil(nlayer/4)}, then fwd_waves {1, 3, 4} at the
  *            best of those;

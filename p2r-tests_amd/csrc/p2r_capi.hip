@@ -62,7 +62,8 @@ __device__ __forceinline__ void load_hit_cached(const float* hb, uint32_t bs, Hi
 // byte offset is fixed for the whole pass and the layer and field go in the scalar offset, so a
 // layer's 9 loads need no address arithmetic.  The active lanes of a wavefront are a prefix of its
 // 64 consecutive tracks (only the batch's end cuts it), so they span at most 63 / bs + 2 blocks:
-// under 2.4 MB of hit rows for bs <= 1024 and nlayer <= 32.  The record count is that span.
+// under 4.8 MB of hit rows for bs <= 1024 and nlayer <= P2R_MAX_NLAYER (64), which the entry points
+// enforce, so the 32-bit record count and offsets cannot wrap.  The record count is that span.
 struct WaveHits {
   __amdgpu_buffer_rsrc_t rs;
   uint32_t voff, field_bytes;
@@ -117,7 +118,7 @@ template <int SEM, bool CHI2, bool EXACT, int MINW>
 __global__ __launch_bounds__(kThreads, MINW) void fwd_uniform_kernel(const float* trk, const float* __restrict__ hits,
                                                                float* out, float* __restrict__ chi2, uint32_t ntrk,
                                                                uint32_t bs, int nlayer, float kfact, int l0, int l1,
-                                                               bool acc, int opts) {
+                                                               bool acc, bool store, int opts) {
   extern __shared__ float cov_lds[];  // [21][blockDim.x], reference semantics only
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ntrk) return;
@@ -177,9 +178,11 @@ __global__ __launch_bounds__(kThreads, MINW) void fwd_uniform_kernel(const float
     }
     cur = prop;
   }
-  float* ob = out + (size_t)blk * 28 * bs + lane;
-  store_trk(ob, bs, cur);
-  reinterpret_cast<int*>(ob)[(size_t)27 * bs] = q;
+  if (store) {
+    float* ob = out + (size_t)blk * 28 * bs + lane;
+    store_trk(ob, bs, cur);
+    reinterpret_cast<int*>(ob)[(size_t)27 * bs] = q;
+  }
   if (CHI2 && !per_layer) chi2[g] = c2;
 }
 
@@ -904,9 +907,13 @@ struct UniformLaunch {
 template <int SEM, bool CHI2, bool EXACT, int MINW>
 void launch_uniform_w(dim3 grid, dim3 block, hipStream_t st, const UniformLaunch& a, const float* in, int l0, int l1,
                       bool acc) {
+  // Reference semantics: only the launch that runs the last layer writes the track (an earlier launch's
+  // result would be overwritten anyway), so trk_in stays intact for every launch even when trk_out
+  // aliases it.
+  const bool store = SEM == P2R_CHAINED || l1 == a.nlayer;
   fwd_uniform_kernel<SEM, CHI2, EXACT, MINW><<<grid, block, SEM == P2R_REFERENCE ? 21 * block.x * sizeof(float) : 0,
                                                st>>>(
-      in, a.H, a.O, a.chi2, a.n, a.bs, a.nlayer, a.kf, l0, l1, acc, a.opts);
+      in, a.H, a.O, a.chi2, a.n, a.bs, a.nlayer, a.kf, l0, l1, acc, store, a.opts);
 }
 template <int SEM, bool CHI2, bool EXACT>
 void launch_uniform_e(int waves, dim3 grid, dim3 block, hipStream_t st, const UniformLaunch& a, const float* in,
@@ -926,7 +933,8 @@ void launch_uniform_p(bool exact, int waves, dim3 grid, dim3 block, hipStream_t 
 }
 // The whole forward: one launch over all layers, or launches of `per_launch`
 // layers (the state passes through trk_out between launches; in reference
-// semantics every launch restarts from trk_in, as every layer does).
+// semantics every launch restarts from trk_in, as every layer does, and only
+// the last launch writes trk_out).
 void launch_uniform(int sem, bool exact, int threads, int per_launch, int waves, hipStream_t st,
                     const UniformLaunch& a) {
   const dim3 grid(grid_for_n(a.n, threads)), block(threads);
@@ -1034,7 +1042,8 @@ int adjoint_plan(p2r_ctx* c, const void* trk_in, const void* hits, const void* t
                  const p2r_params* params, int grad_precision, int32_t checkpoint_stride, bool global,
                  AdjointPlan& pl) {
   if (!trk_in || !hits || !trk_out || !s) return c->fail(P2R_ERR_ARG, "null pointer argument");
-  if (s->nevts < 0 || s->ntrks < 0 || s->nlayer < 1 || s->nlayer > 64 || s->bsize < 1 || s->bsize > 1024)
+  if (s->nevts < 0 || s->ntrks < 0 || s->nlayer < 1 || s->nlayer > P2R_MAX_NLAYER || s->bsize < 1 ||
+      s->bsize > 1024)
     return c->fail(P2R_ERR_SHAPE, "bad shape nevts=%d ntrks=%d nlayer=%d bsize=%d", s->nevts, s->ntrks, s->nlayer,
                    s->bsize);
   if (s->ntrks % s->bsize != 0) return c->fail(P2R_ERR_SHAPE, "ntrks must be a multiple of bsize");
@@ -1045,6 +1054,9 @@ int adjoint_plan(p2r_ctx* c, const void* trk_in, const void* hits, const void* t
     return c->fail(P2R_ERR_ARG, "checkpoint_stride must be 0, 1, 2, 4 or 8");
   const uint64_t ntrk = (uint64_t)s->nevts * (uint64_t)s->ntrks;
   if (ntrk >= (1ull << 31)) return c->fail(P2R_ERR_SHAPE, "too many tracks");
+  // the reverse sweep rereads trk_in after the forward pass has written trk_out
+  if (overlaps(trk_in, ntrk * 112, trk_out, ntrk * 112))
+    return c->fail(P2R_ERR_ARG, "trk_out overlaps trk_in (the adjoint rereads the input tracks)");
   const p2r_params p = params ? *params : p2r_default_params();
   if (!(p.b_scale != 0.f)) return c->fail(P2R_ERR_ARG, "b_scale must be non-zero");
   if (p.precision != P2R_EXACT && p.precision != P2R_FAST) return c->fail(P2R_ERR_ARG, "bad precision %d", p.precision);
@@ -1208,7 +1220,8 @@ int p2r_forward(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out,
   RangeScope range(c, "p2r_forward");
   if (!trk_in || !hits || !trk_out || !s) return c->fail(P2R_ERR_ARG, "null pointer argument");
   if (semantics != P2R_REFERENCE && semantics != P2R_CHAINED) return c->fail(P2R_ERR_ARG, "bad semantics %d", semantics);
-  if (s->nevts < 0 || s->ntrks < 0 || s->nlayer < 1 || s->bsize < 1 || s->bsize > 1024)
+  if (s->nevts < 0 || s->ntrks < 0 || s->nlayer < 1 || s->nlayer > P2R_MAX_NLAYER || s->bsize < 1 ||
+      s->bsize > 1024)
     return c->fail(P2R_ERR_SHAPE, "bad shape nevts=%d ntrks=%d nlayer=%d bsize=%d", s->nevts, s->ntrks, s->nlayer,
                    s->bsize);
   if (s->ntrks % s->bsize != 0)
@@ -1239,7 +1252,8 @@ int p2r_forward_streamed(p2r_ctx* c, const void* host_trk_in, const void* host_h
     return c->fail(P2R_ERR_ARG, "null pointer argument");
   if (semantics != P2R_REFERENCE && semantics != P2R_CHAINED) return c->fail(P2R_ERR_ARG, "bad semantics %d", semantics);
   if (nstreams < 1 || nstreams > 16) return c->fail(P2R_ERR_ARG, "nstreams must be in 1..16");
-  if (s->nevts < 0 || s->ntrks < 0 || s->nlayer < 1 || s->bsize < 1 || s->bsize > 1024 || s->ntrks % s->bsize != 0)
+  if (s->nevts < 0 || s->ntrks < 0 || s->nlayer < 1 || s->nlayer > P2R_MAX_NLAYER || s->bsize < 1 ||
+      s->bsize > 1024 || s->ntrks % s->bsize != 0)
     return c->fail(P2R_ERR_SHAPE, "bad shape nevts=%d ntrks=%d nlayer=%d bsize=%d", s->nevts, s->ntrks, s->nlayer,
                    s->bsize);
   const uint64_t nblk = (uint64_t)s->nevts * (uint64_t)(s->ntrks / s->bsize);

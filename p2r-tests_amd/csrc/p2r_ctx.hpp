@@ -98,6 +98,12 @@ struct p2r_ctx {
   }
 };
 
+// True when the byte ranges [a, a + na) and [b, b + nb) share a byte.
+inline bool overlaps(const void* a, size_t na, const void* b, size_t nb) {
+  const char *x = static_cast<const char*>(a), *y = static_cast<const char*>(b);
+  return na > 0 && nb > 0 && x < y + nb && y < x + na;
+}
+
 // A roctx range for the scope of one entry point (no-op when the context's ranges are off).
 struct RangeScope {
   const Roctx* r;

@@ -18,23 +18,12 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/p2r.h"
+#include "p2r_types.hpp"
 
 namespace p2r {
 
-constexpr int kNiter = 5;  // fixed-point steps of propagateToR, tbb.cpp:655
-
-template <class R> struct TrkT {
-  R par[6];   // x, y, z, 1/pt, phi, theta
-  R cov[21];  // packed lower-triangular symmetric
-};
-template <class R> struct HitT {
-  R pos[3];
-  R cov[6];  // xx, yx, yy, zx, zy, zz
-};
 using Trk = TrkT<float>;
 using Hit = HitT<float>;
-
-__host__ __device__ constexpr int sym(int i, int j) { return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i; }
 
 // Keeps `v` opaque to the optimizer (no code).  Used so values that are
 // loop-invariant only because of reference semantics are recomputed per layer
@@ -46,15 +35,6 @@ __device__ __forceinline__ void opaque(Trk& t) {
 #pragma unroll
   for (int i = 0; i < 21; ++i) opaque(t.cov[i]);
 }
-
-// Non-trivial entries of the propagation Jacobian (errorProp, tbb.cpp:753-807).
-// Rows 3 and 5 are unit rows; J22 = 1; J02 = J05 = J12 = J15 = J42 = J45 = 0.
-template <class R> struct JacT {
-  R j00, j01, j03, j04;
-  R j10, j11, j13, j14;
-  R j20, j21, j23, j24, j25;
-  R j40, j41, j43, j44;
-};
 
 template <class A, class B> __device__ __forceinline__ void convert(const TrkT<A>& a, TrkT<B>& b) {
 #pragma unroll

@@ -38,7 +38,7 @@ void check(int st, p2r_ctx* c, const char* what) {
 
 p2r_shape make_shape(int64_t nevts, int64_t ntrks, int64_t nlayer, int64_t bsize) {
   TORCH_CHECK(nevts >= 0 && ntrks >= 0 && nlayer >= 1 && bsize >= 1 && nevts < (1 << 30) && ntrks < (1 << 30) &&
-                  nlayer < (1 << 16) && bsize <= 1024,
+                  nlayer <= P2R_MAX_NLAYER && bsize <= 1024,
               "bad shape");
   return p2r_shape{(int32_t)nevts, (int32_t)ntrks, (int32_t)nlayer, (int32_t)bsize};
 }

@@ -142,6 +142,9 @@ int p2r_autotune_forward(p2r_ctx* c, const void* trk_in, const void* hits, void*
   if (!c) return P2R_ERR_ARG;
   RangeScope range(c, "p2r_autotune_forward");
   if (!s || reps < 1) return c->fail(P2R_ERR_ARG, "null shape or reps < 1");
+  if (s->nevts > 0 && s->ntrks > 0 &&
+      overlaps(trk_in, (size_t)s->nevts * s->ntrks * 112, trk_out, (size_t)s->nevts * s->ntrks * 112))
+    return c->fail(P2R_ERR_ARG, "trk_out overlaps trk_in (every candidate must start from the same input)");
   const p2r_tuning keep = c->tune;
   hipStream_t st = static_cast<hipStream_t>(stream);
   std::vector<p2r_tuning> cand;
@@ -195,6 +198,11 @@ int p2r_autotune_ragged(p2r_ctx* c, const void* trk_in, const void* hits, void* 
   if (!c) return P2R_ERR_ARG;
   RangeScope range(c, "p2r_autotune_ragged");
   if (reps < 1) return c->fail(P2R_ERR_ARG, "reps < 1");
+  if (blk_offsets && nevts > 0 && bsize > 0 && blk_offsets[nevts] > 0) {
+    const size_t bytes = (size_t)blk_offsets[nevts] * (size_t)bsize * 112;
+    if (overlaps(trk_in, bytes, trk_out, bytes))
+      return c->fail(P2R_ERR_ARG, "trk_out overlaps trk_in (every candidate must start from the same input)");
+  }
   const p2r_tuning keep = c->tune;
   hipStream_t st = static_cast<hipStream_t>(stream);
   struct Cand {

@@ -78,9 +78,12 @@ def _check_host(t, n, name):
         raise ValueError(f"{name}: {size} elements, the shape needs {n}")
 
 
-def _uniform_ok(sh, max_layers=1 << 30):
+P2R_MAX_NLAYER = 64  # include/p2r.h
+
+
+def _uniform_ok(sh):
     """p2r_forward / p2r_forward_adjoint accept this shape (their host checks)."""
-    return (sh.nevts >= 0 and sh.ntrks >= 0 and 1 <= sh.bsize <= 1024 and 1 <= sh.nlayer <= max_layers
+    return (sh.nevts >= 0 and sh.ntrks >= 0 and 1 <= sh.bsize <= 1024 and 1 <= sh.nlayer <= P2R_MAX_NLAYER
             and sh.ntrks % sh.bsize == 0 and sh.nevts * sh.ntrks < (1 << 31))
 
 
@@ -174,7 +177,7 @@ class Context:
 
     def _check_adjoint(self, trk, hits, out, sh, p, gp, cs, chi2, grad_par, grad_hits, grad_kfact, grad_global,
                        what):
-        if not (_uniform_ok(sh, 64) and _params_ok(p) and p.options == 0 and gp in (0, 1, 2)
+        if not (_uniform_ok(sh) and _params_ok(p) and p.options == 0 and gp in (0, 1, 2)
                 and cs in (0, 1, 2, 4, 8)):
             return
         n, L = sh.nevts * sh.ntrks, sh.nlayer
@@ -225,7 +228,7 @@ class Context:
                                       int(tile_rounds), _stream(stream))
         self._check(st, "p2r_forward_ragged")
 
-    def forward_adjoint(self, trk, hits, out, shape, params=None, grad_precision="f32", checkpoint_stride=0,
+    def forward_adjoint(self, trk, hits, out, shape, params=None, grad_precision="mixed", checkpoint_stride=0,
                         chi2=None, grad_par=None, grad_hits=None, grad_kfact=None, grad_global=None, stream=None,
                         precision=None):
         """Chained fit plus reverse-mode gradient of the total chi2 (include/p2r.h p2r_forward_adjoint).
@@ -267,7 +270,7 @@ class Context:
                     "p2r_graph_forward_ragged")
         return Graph(h, self.device)
 
-    def graph_forward_adjoint(self, trk, hits, out, shape, params=None, grad_precision="f32", checkpoint_stride=0,
+    def graph_forward_adjoint(self, trk, hits, out, shape, params=None, grad_precision="mixed", checkpoint_stride=0,
                               chi2=None, grad_par=None, grad_hits=None, grad_kfact=None, grad_global=None,
                               precision=None):
         sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
@@ -320,7 +323,7 @@ class Context:
                                               C.byref(r), _stream(stream)), "p2r_autotune_ragged")
         return r
 
-    def autotune_adjoint(self, trk, hits, out, shape, params=None, grad_precision="f32", chi2=None, grad_par=None,
+    def autotune_adjoint(self, trk, hits, out, shape, params=None, grad_precision="mixed", chi2=None, grad_par=None,
                          grad_hits=None, grad_kfact=None, grad_global=None, precision=None, reps=3, stream=None):
         sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
         p = params if params is not None else default_params(precision or "exact")

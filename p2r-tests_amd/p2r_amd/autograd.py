@@ -97,7 +97,7 @@ class FitFunction(torch.autograd.Function):
         return grad_par, grad_hit, grad_k, None, None, None, None, None, None
 
 
-def fit(par, cov, q, hit_pos, hit_cov, kfact=None, precision="exact", grad_precision="f32", checkpoint_stride=4):
+def fit(par, cov, q, hit_pos, hit_cov, kfact=None, precision="exact", grad_precision="mixed", checkpoint_stride=4):
     """Chained fit with a hand-written adjoint (see the module docstring).  Returns (chi2, par_out, cov_out)."""
     if kfact is None:
         kfact = default_kfact()

@@ -202,5 +202,37 @@ def compare_statistical(gpu_buf, cpu_buf, alt_buf, bs, par64, cov64):
     return res
 
 
+def rel_to(x_buf, ref_buf, bs):
+    """|x - ref| / s per element (n, 27): parameters then covariance, s the reference's natural scale
+    (scales above), over the tracks finite in both."""
+    xp, xc, _ = _arrays(x_buf, bs)
+    rp, rc, _ = _arrays(ref_buf, bs)
+    sp, sc = scales(rp, rc)
+    X, R, S = np.hstack([xp, xc]), np.hstack([rp, rc]), np.hstack([sp, sc])
+    ok = np.isfinite(X).all(axis=1) & np.isfinite(R).all(axis=1)
+    return np.abs(X[ok] - R[ok]) / S[ok]
+
+
+def vs_reference(gpu_buf, ref_buf, alt_buf, bs):
+    """The element-wise 1e-4 bar against the reference's CPU path: fractions of parameter and covariance
+    elements within RTOL of ref_buf, the largest parameter deviation and the worst element; and the same
+    numbers for alt_buf (the reference's algorithm as its own build flags compile it), the yardstick of
+    what another valid FP32 rounding keeps."""
+    def stats(x):
+        r = rel_to(x, ref_buf, bs)
+        if not r.size:
+            return {"tracks": 0}
+        i, k = np.unravel_index(int(np.argmax(r)), r.shape)
+        return {"tracks": int(len(r)), "params_frac_within_1e-4": float((r[:, :6] <= RTOL).mean()),
+                "params_max_rel": float(r[:, :6].max()), "cov_frac_within_1e-4": float((r[:, 6:] <= RTOL).mean()),
+                "frac_within_1e-4": float((r <= RTOL).mean()),
+                "worst_element": {"field": FIELD_NAMES[k], "rel": float(r[i, k])},
+                "per_field_frac_within_1e-4": [float(v) for v in (r <= RTOL).mean(axis=0)]}
+    out = stats(gpu_buf)
+    if alt_buf is not None:
+        out["reference_build_flags"] = stats(alt_buf)
+    return out
+
+
 def summary_fields(res):
     return {k: v for k, v in res.items() if not k.startswith("per_field")}

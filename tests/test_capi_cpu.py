@@ -27,6 +27,16 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_python_constants_match_the_header():
+    """The Python mirror's copies of the header's enum constants carry the header's values."""
+    import p2r_amd
+    text = open(os.path.join(ROOT, "include", "p2r.h")).read()
+    for name in ("P2R_MAX_NLAYER", "P2R_RAGGED_PAIR_STREAM", "P2R_RAGGED_LAYER_ORDER", "P2R_OPT_CHI2_PER_LAYER",
+                 "P2R_GRAD_MIXED", "P2R_ERR_SHAPE"):
+        m = re.search(name + r"\s*=\s*(-?\d+)", text)
+        assert m and int(m.group(1)) == getattr(p2r_amd, name), name
+
+
 def test_library_is_gfx950_code():
     from p2r_amd import _capi
     blob = open(_capi.LIB_PATH, "rb").read()
@@ -177,7 +187,7 @@ def test_python_buffer_checks_reject_host_memory_and_short_buffers():
     ok = lambda *a: _uniform_ok(p2r_amd.p2r_shape(*a))
     assert ok(1, 64, 20, 32) and ok(0, 32, 20, 32)
     assert not ok(1, 65, 20, 32) and not ok(1, 64, 0, 32) and not ok(1, 64, 20, 2048)
-    assert not ok(70000, 65536, 20, 32) and not _uniform_ok(p2r_amd.p2r_shape(1, 64, 65, 32), 64)
+    assert not ok(70000, 65536, 20, 32) and not ok(1, 64, p2r_amd.P2R_MAX_NLAYER + 1, 32) and ok(1, 64, 64, 32)
     offs = np.array([0, 1, 3], np.int64)
     assert _ragged_ok(offs, 20, 32) and not _ragged_ok(offs, 33, 32) and not _ragged_ok(np.array([1, 2]), 20, 32)
     assert not _ragged_ok(np.array([0, 2, 1]), 20, 32)

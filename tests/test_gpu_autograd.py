@@ -39,12 +39,13 @@ def test_extension_is_the_in_tree_build(ag):
 
 
 def test_forward_equals_c_abi_chained(ag, batch):
-    """chi2 and fitted state of the autograd path equal p2r_forward (chained, exact) bit for bit."""
+    """With FP32 gradients, chi2 and fitted state of the autograd path equal p2r_forward (chained, exact)
+    bit for bit (the default, mixed, carries the fit in FP64 and returns it rounded to float)."""
     import p2r_amd
     shape, trk, hits = batch
     par, cov, q, hp, hc = _inputs(ag, shape, trk, hits)
     with torch.no_grad():
-        chi2, par_out, cov_out = ag.fit(par, cov, q, hp, hc)
+        chi2, par_out, cov_out = ag.fit(par, cov, q, hp, hc, grad_precision="f32")
     ctx = p2r_amd.Context(0)
     dt, dh = torch.from_numpy(trk).cuda(), torch.from_numpy(hits).cuda()
     do = torch.empty_like(dt)

@@ -48,7 +48,8 @@ def test_forward_argument_errors(ctx):
     assert L.p2r_forward(ctx._h, P(t), P(h), P(t), None, 0, C.byref(p), None, stream) == p2r_amd.P2R_ERR_ARG
     assert L.p2r_forward(None, P(t), P(h), P(t), C.byref(sh), 0, C.byref(p), None, stream) == p2r_amd.P2R_ERR_ARG
     for bad in (Shape(1, 64, 0, 32), Shape(1, 64, 20, 0), Shape(1, 64, 20, 2048), Shape(-1, 64, 20, 32),
-                Shape(1, 65, 20, 32), Shape(70000, 65536, 20, 32)):
+                Shape(1, 65, 20, 32), Shape(70000, 65536, 20, 32), Shape(1, 64, p2r_amd.P2R_MAX_NLAYER + 1, 32),
+                Shape(1, 64, 60000, 32)):
         assert _status(lambda: ctx.forward(t, h, t, bad)) == p2r_amd.P2R_ERR_SHAPE, bad
     assert _status(lambda: ctx.forward(t, h, t, Shape(1, 64, 20, 32), semantics=5)) == p2r_amd.P2R_ERR_ARG
     for prm in (p2r_amd.default_params(precision=7), p2r_amd.default_params(b_scale=0.0),
@@ -88,15 +89,22 @@ def test_ragged_argument_errors(ctx):
 def test_adjoint_and_graph_argument_errors(ctx):
     import p2r_amd
     t = torch.zeros(28 * 32 * 2, device="cuda")
+    o = torch.zeros(28 * 32 * 2, device="cuda")
     h = torch.zeros(9 * 32 * 65 * 2, device="cuda")
     s20 = Shape(1, 64, 20, 32)
-    assert _status(lambda: ctx.forward_adjoint(t, h, t, Shape(1, 64, 65, 32))) == p2r_amd.P2R_ERR_SHAPE
-    assert _status(lambda: ctx.forward_adjoint(t, h, t, Shape(1, 65, 20, 32))) == p2r_amd.P2R_ERR_SHAPE
-    assert _status(lambda: ctx.forward_adjoint(t, h, t, s20, grad_precision=5)) == p2r_amd.P2R_ERR_ARG
-    assert _status(lambda: ctx.forward_adjoint(t, h, t, s20, checkpoint_stride=3)) == p2r_amd.P2R_ERR_ARG
-    assert _status(lambda: ctx.forward_adjoint(t, h, t, s20, params=p2r_amd.default_params(options=2))) == \
+    assert _status(lambda: ctx.forward_adjoint(t, h, o, Shape(1, 64, 65, 32))) == p2r_amd.P2R_ERR_SHAPE
+    assert _status(lambda: ctx.forward_adjoint(t, h, o, Shape(1, 65, 20, 32))) == p2r_amd.P2R_ERR_SHAPE
+    assert _status(lambda: ctx.forward_adjoint(t, h, o, s20, grad_precision=5)) == p2r_amd.P2R_ERR_ARG
+    assert _status(lambda: ctx.forward_adjoint(t, h, o, s20, checkpoint_stride=3)) == p2r_amd.P2R_ERR_ARG
+    assert _status(lambda: ctx.forward_adjoint(t, h, o, s20, params=p2r_amd.default_params(options=2))) == \
         p2r_amd.P2R_ERR_UNSUPPORTED
-    assert _status(lambda: ctx.graph_forward_adjoint(t, h, t, s20, checkpoint_stride=5)) == p2r_amd.P2R_ERR_ARG
+    # the reverse sweep rereads the input tracks: an output on top of them is refused, whole or in part
+    assert _status(lambda: ctx.forward_adjoint(t, h, t, s20)) == p2r_amd.P2R_ERR_ARG
+    assert b"overlaps" in p2r_amd._capi.lib().p2r_last_error(ctx._h)
+    big = torch.zeros(28 * 32 * 3, device="cuda")
+    assert _status(lambda: ctx.forward_adjoint(big[:28 * 64], h, big[28 * 32:], s20)) == p2r_amd.P2R_ERR_ARG
+    assert _status(lambda: ctx.graph_forward_adjoint(t, h, t, s20)) == p2r_amd.P2R_ERR_ARG
+    assert _status(lambda: ctx.graph_forward_adjoint(t, h, o, s20, checkpoint_stride=5)) == p2r_amd.P2R_ERR_ARG
     assert _status(lambda: ctx.graph_forward_ragged(t, h, t, np.array([0, 1], np.int64), None, 20, 32)) == \
         p2r_amd.P2R_ERR_ARG
     from p2r_amd import _capi
@@ -123,6 +131,23 @@ def test_tuning_streamed_and_repack_errors(ctx):
     b = torch.zeros(28 * 64, device="cuda")
     assert _status(lambda: ctx.repack(a, 48, b, 16, 1, 64, 1, 28)) == p2r_amd.P2R_ERR_SHAPE  # 48 does not divide 64
     ctx.repack(a, 32, b, 16, 1, 64, 1, 28)  # the context still works
+
+
+def test_autotuners_refuse_aliased_tracks(ctx):
+    """Every autotuner candidate must start from the same input: an output buffer on top of the input is
+    refused before anything is timed, and the tuning is left as it was."""
+    import p2r_amd
+    s = Shape(1, 64, 20, 32)
+    t = torch.zeros(28 * 64, device="cuda")
+    h = torch.zeros(9 * 64 * 20, device="cuda")
+    before = ctx.get_tuning()
+    assert _status(lambda: ctx.autotune_forward(t, h, t, s, reps=1)) == p2r_amd.P2R_ERR_ARG
+    assert _status(lambda: ctx.autotune_adjoint(t, h, t, s, reps=1)) == p2r_amd.P2R_ERR_ARG
+    m = torch.full((64,), 0xfffff, dtype=torch.int32, device="cuda")
+    offs = np.array([0, 2], np.int64)
+    assert _status(lambda: ctx.autotune_ragged(t, h, t, offs, m, 20, 32, reps=1)) == p2r_amd.P2R_ERR_ARG
+    after = ctx.get_tuning()
+    assert bytes(before) == bytes(after)
 
 
 def test_python_mirror_checks_buffers_before_launch(ctx):
