@@ -198,6 +198,29 @@ int p2r_forward_adjoint(p2r_ctx* ctx, const void* trk_in, const void* hits, void
                         const p2r_params* params, int grad_precision, int32_t checkpoint_stride, float* chi2,
                         float* grad_par, float* grad_hits, float* grad_kfact, double* grad_global, void* stream);
 
+/* p2r_forward_adjoint without allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, checks and results as p2r_forward_adjoint, but the checkpoints,
+ * chi2 totals and global-sum partials live in a static device array of
+ * P2R_NOALLOC_SCRATCH_BYTES that the library holds on every GPU it runs on
+ * until the process ends (its address is resolved once per device at
+ * p2r_create); the call itself allocates nothing.  A call that needs more
+ * scratch returns P2R_ERR_SHAPE and launches nothing.  Scratch of a call:
+ * (ceil(nlayer/stride) - 1) * 27 * ntrk checkpoint states plus ntrk chi2
+ * totals, of 4 bytes (P2R_GRAD_F32) or 8, plus 8 * (4*nlayer + 1) *
+ * ceil(ntrk/256) bytes when grad_global is requested, each part rounded up to
+ * 256 bytes, plus 256: at most about 150 000 tracks x 20 layers at stride 4
+ * with P2R_GRAD_F32, about 76 000 with P2R_GRAD_MIXED.
+ * Every call on one GPU shares that GPU's array: calls that overlap in time
+ * on one GPU (any stream, thread or context) race and must be serialized by
+ * the caller.  Nothing in the array needs clearing between calls. */
+enum { P2R_NOALLOC_SCRATCH_BYTES = 64 << 20 };
+int p2r_forward_adjoint_noalloc(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_out,
+                                const p2r_shape* shape, const p2r_params* params, int grad_precision,
+                                int32_t checkpoint_stride, float* chi2, float* grad_par, float* grad_hits,
+                                float* grad_kfact, double* grad_global, void* stream);
+
 /* Block-level mirrors of the reference entry points, over nblk Matriplex
  * blocks.  Pointers address the field arrays of the first block; consecutive
  * blocks are `trk_stride` floats apart for track-side arrays and `hit_stride`

@@ -15,9 +15,15 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <new>
 
 using namespace p2r;
+
+// Scratch of p2r_forward_adjoint_noalloc: device memory every GPU that loads the library holds until the
+// process ends.  Host code never uses this name as an address (on the host it is a stand-in, not device
+// memory): it uses the per-device address that resolve_scratch() obtained from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[P2R_NOALLOC_SCRATCH_BYTES];
 
 // An instantiated graph of one captured call (p2r_graph_* in include/p2r.h).
 struct p2r_graph {
@@ -1029,6 +1035,25 @@ void launch_adjoint_p(int gp, unsigned grid, size_t lds, hipStream_t st, const f
 
 inline size_t ck_bytes_aligned(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// Address of `scratch` on each device, resolved once per device with that device current.
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxScratchDevices] = {};
+
+int resolve_scratch(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxScratchDevices) return P2R_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    DeviceGuard g(device);
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return P2R_ERR_HIP;
+    g_scratch[device] = p;
+  }
+  *out = g_scratch[device];
+  return P2R_OK;
+}
+
 // Validated arguments and workspace layout of one adjoint call.
 struct AdjointPlan {
   uint32_t n = 0, bs = 0;
@@ -1196,6 +1221,9 @@ int p2r_create(int device, p2r_ctx** out) {
     return P2R_ERR_HIP;
   }
   c->num_cus = prop.multiProcessorCount;
+  // the no-allocation entry point's static scratch on this device (its status is kept: only that entry
+  // point needs it)
+  c->scratch_status = resolve_scratch(device, &c->scratch);
   c->roctx.load();
   *out = c;
   return P2R_OK;
@@ -1368,6 +1396,30 @@ int p2r_forward_adjoint(p2r_ctx* c, const void* trk_in, const void* hits, void* 
   if (!ws) return c->fail(P2R_ERR_NOMEM, "workspace of %zu bytes", pl.ws_bytes);
   return adjoint_enqueue(c, pl, trk_in, hits, trk_out, chi2, grad_par, grad_hits, grad_kfact, grad_global, ws,
                          static_cast<hipStream_t>(stream));
+}
+
+int p2r_forward_adjoint_noalloc(p2r_ctx* c, const void* trk_in, const void* hits, void* trk_out, const p2r_shape* s,
+                                const p2r_params* params, int grad_precision, int32_t checkpoint_stride, float* chi2,
+                                float* grad_par, float* grad_hits, float* grad_kfact, double* grad_global,
+                                void* stream) {
+  if (!c) return P2R_ERR_ARG;
+  RangeScope range(c, "p2r_forward_adjoint_noalloc");
+  AdjointPlan pl;
+  const int rc = adjoint_plan(c, trk_in, hits, trk_out, s, params, grad_precision, checkpoint_stride,
+                              grad_global != nullptr, pl);
+  if (rc != P2R_OK || pl.n == 0) return rc;
+  if (pl.ws_bytes > (size_t)P2R_NOALLOC_SCRATCH_BYTES)
+    return c->fail(P2R_ERR_SHAPE,
+                   "p2r_forward_adjoint_noalloc: %u tracks x %d layers need %zu bytes of scratch, the static "
+                   "scratch holds %d (fewer tracks per call, a larger checkpoint stride or P2R_GRAD_F32)",
+                   pl.n, pl.nlayer, pl.ws_bytes, (int)P2R_NOALLOC_SCRATCH_BYTES);
+  if (c->scratch_status != P2R_OK || !c->scratch)
+    return c->fail(P2R_ERR_HIP, "p2r_forward_adjoint_noalloc: no static scratch on device %d", c->device);
+  DeviceGuard g(c->device);
+  // every word of the scratch this call reads it writes first (checkpoints, chi2 totals, one partial per
+  // workgroup and global entry): nothing needs clearing between calls
+  return adjoint_enqueue(c, pl, trk_in, hits, trk_out, chi2, grad_par, grad_hits, grad_kfact, grad_global,
+                         static_cast<char*>(c->scratch), static_cast<hipStream_t>(stream));
 }
 
 // ---- HIP graphs ------------------------------------------------------------------------------------------------

@@ -65,6 +65,9 @@ struct p2r_ctx {
   // Workspace owned by the context (device memory), grown on demand.
   void* ws = nullptr;
   size_t ws_bytes = 0;
+  // This device's copy of the library's static scratch (p2r_forward_adjoint_noalloc), resolved at creation.
+  void* scratch = nullptr;
+  int scratch_status = P2R_ERR_HIP;
 
   int fail(int status, const char* fmt, ...) {
     va_list ap;

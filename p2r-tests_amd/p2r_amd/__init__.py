@@ -79,6 +79,7 @@ def _check_host(t, n, name):
 
 
 P2R_MAX_NLAYER = 64  # include/p2r.h
+P2R_NOALLOC_SCRATCH_BYTES = 64 << 20  # include/p2r.h: static scratch of p2r_forward_adjoint_noalloc
 
 
 def _uniform_ok(sh):
@@ -243,6 +244,21 @@ class Context:
                                        int(checkpoint_stride), _ptr(chi2), _ptr(grad_par), _ptr(grad_hits),
                                        _ptr(grad_kfact), _ptr(grad_global), _stream(stream))
         self._check(st, "p2r_forward_adjoint")
+
+    def forward_adjoint_noalloc(self, trk, hits, out, shape, params=None, grad_precision="mixed",
+                                checkpoint_stride=0, chi2=None, grad_par=None, grad_hits=None, grad_kfact=None,
+                                grad_global=None, stream=None, precision=None):
+        """forward_adjoint with its scratch in the library's static device array (include/p2r.h
+        p2r_forward_adjoint_noalloc): no allocation; calls on one GPU must not overlap in time."""
+        sh = p2r_shape(*shape.tuple()) if isinstance(shape, Shape) else shape
+        p = params if params is not None else default_params(precision or "exact")
+        gp = GRAD_PRECISION[grad_precision] if isinstance(grad_precision, str) else int(grad_precision)
+        self._check_adjoint(trk, hits, out, sh, p, gp, int(checkpoint_stride), chi2, grad_par, grad_hits, grad_kfact,
+                            grad_global, "p2r_forward_adjoint_noalloc")
+        st = lib().p2r_forward_adjoint_noalloc(self._h, _ptr(trk), _ptr(hits), _ptr(out), C.byref(sh), C.byref(p), gp,
+                                               int(checkpoint_stride), _ptr(chi2), _ptr(grad_par), _ptr(grad_hits),
+                                               _ptr(grad_kfact), _ptr(grad_global), _stream(stream))
+        self._check(st, "p2r_forward_adjoint_noalloc")
 
     # ---- HIP graphs of one call (include/p2r.h p2r_graph_*) ----
     def graph_forward(self, trk, hits, out, shape, semantics="reference", params=None, chi2=None, precision=None):

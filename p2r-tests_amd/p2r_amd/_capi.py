@@ -72,6 +72,8 @@ def lib():
                                     vp], C.c_int),
             "p2r_forward_adjoint": ([vp, vp, vp, vp, C.POINTER(p2r_shape), C.POINTER(p2r_params), C.c_int, i32, vp,
                                      vp, vp, vp, vp, vp], C.c_int),
+            "p2r_forward_adjoint_noalloc": ([vp, vp, vp, vp, C.POINTER(p2r_shape), C.POINTER(p2r_params), C.c_int,
+                                             i32, vp, vp, vp, vp, vp, vp], C.c_int),
             "p2r_propagateToR": ([vp, vp, vp, vp, vp, vp, vp, i32, i64, i64, i64, f32, vp], C.c_int),
             "p2r_KalmanUpdate": ([vp, vp, vp, vp, vp, i32, i64, i64, i64, vp], C.c_int),
             "p2r_gen_reference": ([vp, vp, C.POINTER(p2r_shape)], C.c_int),
