@@ -139,7 +139,7 @@ int p2r_forward(p2r_ctx* ctx, const void* trk_in, const void* hits, void* trk_ou
  *     (no idle lanes; each lane reads its own layers, so hit loads gather
  *     from many cache lines);
  *   P2R_RAGGED_PAIR_STREAM (-2): each wavefront keeps the state of a tile of
- *     128 slots in LDS and deals the tile's present (layer, slot) pairs, in
+ *     112 slots in LDS and deals the tile's present (layer, slot) pairs, in
  *     layer-major order, 64 per step to its lanes (no idle lanes; the hit rows
  *     of a step belong to at most two layers, so loads stay near-coalesced);
  *   P2R_RAGGED_AUTO (0): the library's choice (the context's ragged_schedule).

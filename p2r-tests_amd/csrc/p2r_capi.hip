@@ -417,16 +417,20 @@ __global__ __launch_bounds__(kThreads) void fwd_ragged_layer_kernel(const float*
 // whole pass.  The tile's present (layer, slot) pairs are taken in layer-major
 // order, 64 at a time, and dealt to the lanes: no lane idles on an absent
 // layer, and the lanes of one step read the hit rows of at most two layers of
-// the tile's blocks (near-coalesced).  A step takes its pairs from a window of
-// kPsTile consecutive (layer, slot) positions, so no slot occurs twice in one
-// step; steps run in order within the wavefront, so a track's layers are
+// the tile's blocks (near-coalesced).  A step takes its pairs from a window that
+// holds each of the tile's slots once (slots s0.. of one layer, then slots ..s0-1
+// of the next), so no slot occurs twice in one step; steps run in order within the wavefront, so a track's layers are
 // updated in increasing order exactly as in the one-lane-per-track kernels
 // (same per-lane arithmetic, same chi2 summation order: bit-identical).
 // Wavefronts are independent (no workgroup barrier); LDS is private per
 // wavefront.
-constexpr int kPsTile = 128;   // slots per wavefront tile (2 per lane)
+// 112 slots: 12.5 KB of state per wavefront without chi2, so 12 wavefronts fit a CU's LDS (128 slots fit
+// 11; measured 1-2 % slower), while a window still holds 70 pairs at the uneven workload's density.
+constexpr int kPsTile = 112;   // slots per wavefront tile (lane l holds slots l and l + 64, those below the tile)
+constexpr int kPsPos = 128;    // position stride: position = layer * kPsPos + slot (slot in 7 bits)
 constexpr int kPsWords = 29;   // per slot: 27 state floats, charge, chi2 sum (when returned)
-constexpr int kPsWaves = 1;    // wavefronts per workgroup (one: 14.6 KB of LDS each, 11 per CU)
+constexpr int kPsWaves = 1;    // wavefronts per workgroup
+static_assert(kPsTile > 64 && kPsTile <= kPsPos, "a tile holds 65..128 slots");
 constexpr uint32_t kPsNone = 0xffffffffu;
 
 // Orders this wavefront's LDS accesses across lanes (LDS operations of one
@@ -437,7 +441,7 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One step of the schedule from window start `c` (position = layer*kPsTile + slot).
+// One step of the schedule from window start `c` (position = layer*kPsPos + slot, slot < kPsTile).
 // Returns this lane's pair code (slot | layer << 7 | last-layer flag << 12 |
 // lane-in-block << 13 | block - the tile's first block << 23) or kPsNone, and
 // the next window start in `next` (wave-uniform).  m: mask of this lane's two
@@ -446,9 +450,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 __device__ __forceinline__ uint32_t ps_step(uint32_t c, const uint32_t m[2], const uint32_t sc[2], int lane,
                                            uint32_t* tbl, uint32_t& next) {
   c = __builtin_amdgcn_readfirstlane(c);  // wave-uniform: keeps the window arithmetic scalar
-  const uint32_t L0 = c >> 7, s0 = c & (kPsTile - 1);  // L0 < nlayer <= 32
-  // Window order: slots s0..127 at layer L0, then slots 0..s0-1 at layer L0 + 1.  A slot is present
-  // in the window if its mask has the layer the window gives it.
+  const uint32_t L0 = c >> 7, s0 = c & (kPsPos - 1);  // L0 < nlayer <= 32, s0 < kPsTile
+  // Window order: slots s0..kPsTile-1 at layer L0, then slots 0..s0-1 at layer L0 + 1.  A slot is present
+  // in the window if its mask has the layer the window gives it (lane-slots at or above the tile have
+  // mask 0).
   bool first[2], pres[2];
   uint64_t P[2];
 #pragma unroll
@@ -476,7 +481,7 @@ __device__ __forceinline__ uint32_t ps_step(uint32_t c, const uint32_t m[2], con
   const uint32_t take = total < 64 ? total : 64u;
   const uint32_t code = (uint32_t)lane < take ? tbl[lane] : kPsNone;
   const uint32_t t64 = tbl[64];
-  next = __builtin_amdgcn_readfirstlane(total > 64 ? (t64 & 0xfffu) : c + kPsTile);
+  next = __builtin_amdgcn_readfirstlane(total > 64 ? (t64 & 0xfffu) : c + kPsPos);
   wave_lds_sync();
   return code;
 }
@@ -508,7 +513,7 @@ __global__ __launch_bounds__(64 * kPsWaves) void fwd_ragged_stream_kernel(const 
     b[r] = g / bs;
     ln[r] = g - b[r] * bs;
     sc[r] = slot | (ln[r] << 13) | ((b[r] - blk0) << 23);
-    if (g < nslot) {
+    if (slot < kPsTile && g < nslot) {
       const float* tb = trk + (size_t)b[r] * 28 * bs + ln[r];
 #pragma unroll
       for (int f = 0; f < 28; ++f) st[f][slot] = tb[(size_t)f * bs];
@@ -542,7 +547,7 @@ __global__ __launch_bounds__(64 * kPsWaves) void fwd_ragged_stream_kernel(const 
   };
   auto process = [&](uint32_t code, const Hit& h) {
     if (code != kPsNone) {
-      const uint32_t slot = code & (kPsTile - 1);
+      const uint32_t slot = code & (kPsPos - 1);
       Trk src;
 #pragma unroll
       for (int f = 0; f < 6; ++f) src.par[f] = st[f][slot];
@@ -579,7 +584,7 @@ __global__ __launch_bounds__(64 * kPsWaves) void fwd_ragged_stream_kernel(const 
   };
   // Steps alternate between two hit registers sets: the next step's pairs and hit loads are issued
   // before the current step's arithmetic, with no copies between them.
-  const uint32_t cend = (uint32_t)nlayer * kPsTile;
+  const uint32_t cend = (uint32_t)nlayer * kPsPos;
   uint32_t cn = 0;
   uint32_t code_a = ps_step(0u, m, sc, lane, tbl, cn), code_b = kPsNone;
   Hit ha, hb;
@@ -603,7 +608,7 @@ __global__ __launch_bounds__(64 * kPsWaves) void fwd_ragged_stream_kernel(const 
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
     const uint32_t slot = (uint32_t)lane + 64u * r, g = base + slot;
-    if (g >= nslot) continue;
+    if (slot >= kPsTile || g >= nslot) continue;  // another tile's slot, or past the batch
     if (SEM == P2R_CHAINED || m[r] == 0u) {  // chained result, or a track with no present layer copied through
       float* ob = out + (size_t)b[r] * 28 * bs + ln[r];
 #pragma unroll

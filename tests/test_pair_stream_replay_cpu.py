@@ -25,10 +25,16 @@ bit 31 is set into 0xffffffff........, an address no process maps (a GPU memory 
 load).  That is the defect an intermediate build of the pair stream had; the committed join goes through
 uint32_t first.
 """
+import os
+import re
+
 import numpy as np
 import pytest
 
-TILE = 128
+_SRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "p2r-tests_amd", "csrc",
+                    "p2r_capi.hip")
+TILE = int(re.search(r"constexpr int kPsTile = (\d+);", open(_SRC).read()).group(1))  # the kernel's tile
+POS = 128   # position stride: position = layer * POS + slot
 NONE = 0xFFFFFFFF
 
 
@@ -49,17 +55,19 @@ def relayout_masks(mask_src, counts, bs_src, bs_dst):
     return out, od
 
 
-def replay(mask, nslot, bs, nlayer, chi2_per_layer=True, guard_presence=True):
-    """Runs the schedule over all tiles; returns (steps taken by the longest tile, pairs dealt).
-    guard_presence=False replays a variant whose lanes write the table whether or not they have a pair
-    (a planted mistake, for the test that shows the checks can fail)."""
+def replay(mask, nslot, bs, nlayer, chi2_per_layer=True, guard_presence=True, tile=TILE):
+    """Runs the schedule over all tiles of `tile` slots (at most 128: lane l holds slots l and l + 64;
+    positions are layer * 128 + slot whatever the tile); returns (steps taken by the longest tile, pairs
+    dealt).  guard_presence=False replays a variant whose lanes write the table whether or not they have
+    a pair (a planted mistake, for the test that shows the checks can fail)."""
     assert 1 <= nlayer <= 32 and 1 <= bs <= 1024 and nslot % bs == 0  # whole Matriplex blocks
+    assert 64 < tile <= POS
     layer_bits = np.uint64(0xFFFFFFFF if nlayer >= 32 else (1 << nlayer) - 1)
-    ntiles = (nslot + TILE - 1) // TILE
-    base = np.arange(ntiles, dtype=np.int64) * TILE
-    slot = np.arange(TILE, dtype=np.int64)
+    ntiles = (nslot + tile - 1) // tile
+    base = np.arange(ntiles, dtype=np.int64) * tile
+    slot = np.arange(POS, dtype=np.int64)          # the 128 lane-slots; those >= tile hold nothing
     g = base[:, None] + slot[None, :]
-    valid = g < nslot
+    valid = (g < nslot) & (slot[None, :] < tile)
     m = np.where(valid, mask[np.minimum(g, nslot - 1)].astype(np.uint64) & layer_bits, 0).astype(np.uint64)
     blk = g // bs
     ln = g - blk * bs
@@ -68,15 +76,15 @@ def replay(mask, nslot, bs, nlayer, chi2_per_layer=True, guard_presence=True):
     # the code fields must hold their values: slot 7 bits, lane-in-block 10, relative block 9
     assert ln.max() < 1024 and rb.max() < 512
     sc = (slot[None, :] | (ln << 13) | (rb << 23)).astype(np.uint64)
-    blk_last = np.minimum(base + TILE - 1, nslot - 1) // bs
+    blk_last = np.minimum(base + tile - 1, nslot - 1) // bs
     row_bytes, field_bytes = 36 * bs, 4 * bs
     records = (blk_last - blk0 + 1) * nlayer * row_bytes          # buffer resource size (bytes)
     assert records.max() < 2 ** 31                                 # an int in make_buffer_rsrc
     kW = 29                                                        # LDS words per slot (chi2 returned)
     c = np.zeros(ntiles, np.int64)                                 # window start, per tile
-    done = np.zeros((ntiles, TILE), np.uint64)                     # processed layers per slot
-    top = np.full((ntiles, TILE), -1, np.int64)                    # highest processed layer per slot
-    cend = nlayer * TILE
+    done = np.zeros((ntiles, POS), np.uint64)                      # processed layers per slot
+    top = np.full((ntiles, POS), -1, np.int64)                     # highest processed layer per slot
+    cend = nlayer * POS
     steps = pairs = 0
     tiles = np.arange(ntiles)
     while True:
@@ -86,7 +94,8 @@ def replay(mask, nslot, bs, nlayer, chi2_per_layer=True, guard_presence=True):
         steps += 1
         A = tiles[act]
         cA = c[A]
-        L0, s0 = cA >> 7, cA & (TILE - 1)
+        L0, s0 = cA >> 7, cA & (POS - 1)
+        assert s0.max() < tile
         assert L0.max() < nlayer
         first = slot[None, :] >= s0[:, None]
         sh = np.where(first, L0[:, None], L0[:, None] + 1).astype(np.uint64)
@@ -114,14 +123,14 @@ def replay(mask, nslot, bs, nlayer, chi2_per_layer=True, guard_presence=True):
         assert np.all(written[:, :64][dealt] == 1), "a lane reads a table entry not written in this step"
         more = total > 64
         assert np.all(written[more, 64] == 1), "tbl[64] read without a write in this step"
-        nxt = np.where(more, (tbl[:, 64] & np.uint64(0xFFF)).astype(np.int64), cA + TILE)
+        nxt = np.where(more, (tbl[:, 64] & np.uint64(0xFFF)).astype(np.int64), cA + POS)
         assert np.all(nxt > cA), "the window start does not advance"
         # decode the dealt codes exactly as the kernel does
         di, dl = np.nonzero(dealt)
         cd = tbl[di, dl]
         assert np.all(cd != NONE)
         t = A[di]
-        s_ = (cd & np.uint64(TILE - 1)).astype(np.int64)
+        s_ = (cd & np.uint64(POS - 1)).astype(np.int64)
         Ld = ((cd >> np.uint64(7)) & np.uint64(31)).astype(np.int64)
         lastd = ((cd >> np.uint64(12)) & np.uint64(1)).astype(bool)
         lnd = ((cd >> np.uint64(13)) & np.uint64(1023)).astype(np.int64)
@@ -132,12 +141,13 @@ def replay(mask, nslot, bs, nlayer, chi2_per_layer=True, guard_presence=True):
         assert np.all(((m[t, s_] >> Ld.astype(np.uint64)) & np.uint64(1)) == 1), "a pair of an absent layer"
         assert np.all(((done[t, s_] >> Ld.astype(np.uint64)) & np.uint64(1)) == 0), "a pair dealt twice"
         assert np.all(Ld > top[t, s_]), "layers of a slot out of order"
-        key = t * TILE + s_
+        assert np.all(s_ < tile), "a slot outside the tile"
+        key = t * POS + s_
         assert len(np.unique(key)) == len(key), "a slot twice in one step"
         assert np.all(lnd == gs % bs) and np.all(rbd == gs // bs - blk0[t]), "code fields of another slot"
         assert np.all(lastd == ((m[t, s_] >> Ld.astype(np.uint64)) <= 1)), "wrong last-layer flag"
         # addresses: LDS column, hit row (9 fields) inside the resource, chi2 word, output block
-        assert s_.max(initial=0) < TILE and (kW - 1) * TILE + s_.max(initial=0) < kW * TILE
+        assert (kW - 1) * tile + s_.max(initial=0) < kW * tile   # LDS state word st[f][slot], [kW][tile]
         off = (rbd * nlayer + Ld) * row_bytes + ((cd >> np.uint64(11)) & np.uint64(0xFFC)).astype(np.int64)
         assert np.all(off + 8 * field_bytes + 4 <= records[t]), "hit load past the tile's rows"
         abs_off = ((blk0[t] + rbd) * nlayer + Ld) * row_bytes + 4 * lnd
@@ -175,11 +185,13 @@ def ragged_500():
     return counts, mask
 
 
+@pytest.mark.parametrize("tile", sorted({TILE, 112, 128}))
 @pytest.mark.parametrize("nlayer", [1, 7, 20, 31, 32])
 @pytest.mark.parametrize("density", [0.0, 0.05, 0.5, 0.95, 1.0])
 @pytest.mark.parametrize("bsize", [1, 8, 1024])
-def test_pair_stream_replay_mask_densities(nlayer, density, bsize):
-    """Masks from empty to full, 32-bit masks, tiles cut by the end of the batch and by blocks."""
+def test_pair_stream_replay_mask_densities(nlayer, density, bsize, tile):
+    """Masks from empty to full, 32-bit masks, tiles cut by the end of the batch and by blocks, at the
+    kernel's tile size and at 112 and 128 slots."""
     counts = np.array([40, 17, 0, 33, 300], np.int32)
     offs = _layout(counts, bsize)
     nslot = int(offs[-1]) * bsize
@@ -187,7 +199,7 @@ def test_pair_stream_replay_mask_densities(nlayer, density, bsize):
     bits = rng.random((nslot, nlayer)) < density
     mask = (bits * (1 << np.arange(nlayer, dtype=np.uint64))).sum(1).astype(np.uint32)
     mask[rng.random(nslot) < 0.1] = 0  # padding-like lanes
-    replay(mask, nslot, bsize, nlayer)
+    replay(mask, nslot, bsize, nlayer, tile=tile)
 
 
 def test_replay_catches_a_missing_presence_guard():
