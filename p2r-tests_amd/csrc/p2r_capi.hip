@@ -627,8 +627,9 @@ __global__ __launch_bounds__(64 * kPsWaves) void fwd_ragged_stream_kernel(const 
 // Forward + adjoint of the chained fit.  One lane per track.  Forward pass in
 // the state precision RS, checkpoints every CS layers (checkpoint 0 is the
 // input track, not stored); reverse sweep per segment: the segment's layer
-// inputs are recomputed from its checkpoint (registers), then each layer's
-// values are recomputed and back-propagated in the layer precision.
+// inputs are recomputed from its checkpoint (into LDS or a global buffer, see
+// SegStore), then each layer's values are recomputed and back-propagated in
+// the layer precision.
 constexpr int kAdjThreads = 256;
 
 template <class R, bool EXACT> struct MathOf;
@@ -717,20 +718,40 @@ __global__ __launch_bounds__(kAdjThreads, MINW) void adjoint_fwd_kernel(
   c2_total[g] = c2;
 }
 
-// Reverse sweep of the adjoint, after adjoint_fwd_kernel: per segment, reload the checkpoint, recompute
-// the segment's layer inputs (registers), then per layer recompute its values and back-propagate.
-template <class RS, class RL, bool EXACT, int CS>
-__global__ __launch_bounds__(kAdjThreads) void adjoint_kernel(
+// Where the reverse sweep keeps the recomputed layer inputs of a segment between its forward recompute and
+// their reverse: the inputs of layers l0+1 .. l0+len-2 (CS-2 states at most; the last one stays in
+// registers, the first is the checkpoint, read again).  In LDS when RW workgroups of them fit beside the
+// global-sum partials, else in a global buffer [slot][field][track] of the workspace.
+// Register budget of the reverse sweep: 2 waves per SIMD with the FP32 reverse, 1 with the FP64 one.
+template <class RL> constexpr int kRevWaves = sizeof(RL) == sizeof(float) ? 2 : 1;
+constexpr size_t kAdjPartialsLdsMax = (size_t)(kAdjThreads / 64) * (4 * P2R_MAX_NLAYER + 1) * sizeof(double);
+template <class RS, int CS, int RW> struct SegStore {
+  static constexpr int kSlots = CS > 2 ? CS - 2 : 0;
+  static constexpr size_t kLdsBytes = (size_t)kSlots * 27 * sizeof(RS) * kAdjThreads;
+  // within 64 KB per workgroup, and RW workgroups (RW waves per SIMD) within the CU's 160 KB
+  static constexpr bool kInLds = kSlots == 0 || (kLdsBytes + kAdjPartialsLdsMax <= 64 * 1024 &&
+                                                 (size_t)RW * (kLdsBytes + kAdjPartialsLdsMax) <= 160 * 1024);
+  static size_t global_bytes(uint64_t ntrk) { return kInLds ? 0 : (size_t)kSlots * 27 * sizeof(RS) * ntrk; }
+};
+
+// Reverse sweep of the adjoint, after adjoint_fwd_kernel: per segment, from the last, reload the checkpoint,
+// recompute the segment's layer inputs, then per layer from the last recompute its values and back-propagate.
+// The layer loops are not unrolled (one copy of each layer body in the code).  RW is the register budget in
+// waves per SIMD.
+template <class RS, class RL, bool EXACT, int CS, int RW = kRevWaves<RL>>
+__global__ __launch_bounds__(kAdjThreads, RW) void adjoint_kernel(
     const float* __restrict__ trk, const float* __restrict__ hits, const RS* __restrict__ c2_total,
     float* __restrict__ gpar, float* __restrict__ ghit, float* __restrict__ gk, double* __restrict__ partials,
-    const RS* __restrict__ ckpt, uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
+    const RS* __restrict__ ckpt, RS* __restrict__ seg_global, uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
   using MS = MathOf<RS, EXACT>;
   // The segment recomputation runs in the requested arithmetic (the layer inputs of the forward pass);
   // the reverse sweep always in the fused instance: gradients have no reference rounding to reproduce,
   // and they are checked against FP64 to a tolerance.
   using ML = MathOf<RL, false>;
+  using SS = SegStore<RS, CS, RW>;
   constexpr int kWaves = kAdjThreads / 64;
-  extern __shared__ double lds_global[];  // [kWaves][4*nlayer + 1] when partials != nullptr
+  // dynamic LDS: [kWaves][4*nlayer + 1] doubles when partials != nullptr, then the segment states when in LDS
+  extern __shared__ double lds_global[];
   const uint32_t g = blockIdx.x * kAdjThreads + threadIdx.x;
   const bool active = g < ntrk;
   const uint32_t gg = active ? g : ntrk - 1;  // inactive lanes recompute a valid track, write nothing
@@ -742,10 +763,47 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_kernel(
   const float k = (float)q * kf;
   const int nseg = (nlayer + CS - 1) / CS;
   const RS c2 = c2_total[gg];
+  const int nglob = 4 * nlayer + 1;
+  RS* seg_lds = reinterpret_cast<RS*>(lds_global + (partials ? kWaves * nglob : 0));
+  // slot e of the current lane: LDS [slot][field][thread] or global [slot][field][track]
+  auto seg_at = [&](int slot, int f) -> RS& {
+    if constexpr (SS::kInLds) return seg_lds[((size_t)slot * 27 + f) * kAdjThreads + threadIdx.x];
+    else return seg_global[((size_t)slot * 27 + f) * ntrk + gg];
+  };
+  auto seg_put = [&](int slot, const TrkT<RS>& t) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) seg_at(slot, f) = t.par[f];
+#pragma unroll
+    for (int f = 0; f < 21; ++f) seg_at(slot, 6 + f) = t.cov[f];
+  };
+  auto seg_get = [&](int slot, TrkT<RS>& t) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) t.par[f] = seg_at(slot, f);
+#pragma unroll
+    for (int f = 0; f < 21; ++f) t.cov[f] = seg_at(slot, 6 + f);
+  };
+  // input of the segment's first layer: the track itself or a checkpoint.  Read again for the segment's first
+  // layer: the addresses pass through an opaque copy so that the compiler reloads instead of keeping the
+  // values of the first read live through the whole segment.
+  auto seg_start = [&](int s, TrkT<RS>& t) {
+    if (s == 0) {
+      const float* tp = tb;
+      asm volatile("" : "+v"(tp));
+      Trk in_f;
+      load_trk(tp, bs, in_f);
+      convert(in_f, t);
+    } else {
+      const RS* cp = ckpt + ((size_t)(s - 1) * 27) * ntrk + gg;
+      asm volatile("" : "+v"(cp));
+#pragma unroll
+      for (int f = 0; f < 6; ++f) t.par[f] = cp[(size_t)f * ntrk];
+#pragma unroll
+      for (int f = 0; f < 21; ++f) t.cov[f] = cp[(size_t)(6 + f) * ntrk];
+    }
+  };
   // ---- reverse sweep ----
   // a track whose fit diverged (non-finite total chi2) adds nothing to the global sums
   const bool in_global = active && isfinite((double)c2);
-  const int nglob = 4 * nlayer + 1;
   const int wave = threadIdx.x >> 6, wl = threadIdx.x & 63;
   TrkT<RL> bar;
 #pragma unroll
@@ -754,53 +812,47 @@ __global__ __launch_bounds__(kAdjThreads) void adjoint_kernel(
   for (int f = 0; f < 21; ++f) bar.cov[f] = 0;
   RL kbar = 0;
   for (int s = nseg - 1; s >= 0; --s) {
-    TrkT<RS> st[CS];
-    if (s == 0) {
-      Trk in_f;
-      load_trk(tb, bs, in_f);
-      convert(in_f, st[0]);
-    } else {
-      const RS* cp = ckpt + ((size_t)(s - 1) * 27) * ntrk + gg;
-#pragma unroll
-      for (int f = 0; f < 6; ++f) st[0].par[f] = cp[(size_t)f * ntrk];
-#pragma unroll
-      for (int f = 0; f < 21; ++f) st[0].cov[f] = cp[(size_t)(6 + f) * ntrk];
-    }
     const int l0 = s * CS, len = min(CS, nlayer - l0);
-#pragma unroll
-    for (int j = 1; j < CS; ++j) {
-      if (j < len) {
-        HitT<RS> h;
-        load_hit_as(hb + (l0 + j - 1) * hstride, bs, h);
-        MS::fwd(st[j - 1], (RS)k, h, st[j]);
-      }
+    TrkT<RS> cur;
+    seg_start(s, cur);
+    // inputs of layers l0+1 .. l0+len-1: the last stays in `cur`, the others go to the segment store
+#pragma nounroll
+    for (int j = 1; j < len; ++j) {
+      if (j >= 2) seg_put(j - 2, cur);
+      HitT<RS> h;
+      load_hit_as(hb + (l0 + j - 1) * hstride, bs, h);
+      TrkT<RS> nxt;
+      MS::fwd(cur, (RS)k, h, nxt);
+      cur = nxt;
     }
+#pragma nounroll
+    for (int j = len - 1; j >= 0; --j) {
+      if (j < len - 1) {
+        if (j == 0) seg_start(s, cur);
+        else seg_get(j - 1, cur);
+      }
+      const int l = l0 + j;
+      HitT<RL> h;
+      load_hit_as(hb + l * hstride, bs, h);
+      TrkT<RL> in_l;
+      convert(cur, in_l);
+      RL hbar[3];
+      ML::adj(in_l, (RL)k, h, bar, kbar, hbar);
+      if (active && ghit) {
+        float* gh = ghit + ((size_t)blk * nlayer + l) * 3 * bs + lane;
 #pragma unroll
-    for (int j = CS - 1; j >= 0; --j) {
-      if (j < len) {
-        const int l = l0 + j;
-        HitT<RL> h;
-        load_hit_as(hb + l * hstride, bs, h);
-        TrkT<RL> in_l;
-        convert(st[j], in_l);
-        RL hbar[3];
-        ML::adj(in_l, (RL)k, h, bar, kbar, hbar);
-        if (active && ghit) {
-          float* gh = ghit + ((size_t)blk * nlayer + l) * 3 * bs + lane;
-#pragma unroll
-          for (int c = 0; c < 3; ++c) gh[(size_t)c * bs] = (float)hbar[c];
-        }
-        if (partials) {
-          const double hx = h.pos[0], hy = h.pos[1];
-          const double a = in_global ? (double)hbar[0] : 0.0, b = in_global ? (double)hbar[1] : 0.0,
-                       cz = in_global ? (double)hbar[2] : 0.0;
-          const double v0 = wave_sum(a), v1 = wave_sum(b), v2 = wave_sum(cz), v3 = wave_sum(-hy * a + hx * b);
-          if (wl == 0) {
-            lds_global[wave * nglob + 4 * l + 0] = v0;
-            lds_global[wave * nglob + 4 * l + 1] = v1;
-            lds_global[wave * nglob + 4 * l + 2] = v2;
-            lds_global[wave * nglob + 4 * l + 3] = v3;
-          }
+        for (int c = 0; c < 3; ++c) gh[(size_t)c * bs] = (float)hbar[c];
+      }
+      if (partials) {
+        const double hx = h.pos[0], hy = h.pos[1];
+        const double a = in_global ? (double)hbar[0] : 0.0, b = in_global ? (double)hbar[1] : 0.0,
+                     cz = in_global ? (double)hbar[2] : 0.0;
+        const double v0 = wave_sum(a), v1 = wave_sum(b), v2 = wave_sum(cz), v3 = wave_sum(-hy * a + hx * b);
+        if (wl == 0) {
+          lds_global[wave * nglob + 4 * l + 0] = v0;
+          lds_global[wave * nglob + 4 * l + 1] = v1;
+          lds_global[wave * nglob + 4 * l + 2] = v2;
+          lds_global[wave * nglob + 4 * l + 3] = v3;
         }
       }
     }
@@ -1013,29 +1065,43 @@ int launch_ragged_s(bool chi2_on, bool exact, int rounds, p2r_ctx* c, hipStream_
 
 template <class RS, class RL, bool EXACT, int CS>
 void launch_adjoint_t(unsigned grid, size_t lds, hipStream_t st, const float* T, const float* H, float* O, float* chi2,
-                      float* gpar, float* ghit, float* gk, double* partials, void* ckpt, void* c2_total, uint32_t ntrk,
-                      uint32_t bs, int nlayer, float kf) {
+                      float* gpar, float* ghit, float* gk, double* partials, void* ckpt, void* c2_total, void* seg,
+                      uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
   // the forward pass at the forward kernels' register budget (3 waves per SIMD in FP32)
   constexpr int kFwdWaves = sizeof(RS) == sizeof(float) ? 3 : 1;
   adjoint_fwd_kernel<RS, EXACT, CS, kFwdWaves><<<grid, kAdjThreads, 0, st>>>(
       T, H, O, chi2, static_cast<RS*>(ckpt), static_cast<RS*>(c2_total), ntrk, bs, nlayer, kf);
-  adjoint_kernel<RS, RL, EXACT, CS><<<grid, kAdjThreads, lds, st>>>(T, H, static_cast<const RS*>(c2_total), gpar,
-                                                                    ghit, gk, partials, static_cast<const RS*>(ckpt),
-                                                                    ntrk, bs, nlayer, kf);
+  using SS = SegStore<RS, CS, kRevWaves<RL>>;
+  const size_t lds_all = lds + (SS::kInLds ? SS::kLdsBytes : 0);
+  adjoint_kernel<RS, RL, EXACT, CS><<<grid, kAdjThreads, lds_all, st>>>(
+      T, H, static_cast<const RS*>(c2_total), gpar, ghit, gk, partials, static_cast<const RS*>(ckpt),
+      static_cast<RS*>(seg), ntrk, bs, nlayer, kf);
 }
 template <bool EXACT, int CS>
 void launch_adjoint_p(int gp, unsigned grid, size_t lds, hipStream_t st, const float* T, const float* H, float* O,
                       float* chi2, float* gpar, float* ghit, float* gk, double* partials, void* ckpt, void* c2_total,
-                      uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
+                      void* seg, uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
   if (gp == P2R_GRAD_F32)
     launch_adjoint_t<float, float, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt, c2_total,
-                                              ntrk, bs, nlayer, kf);
+                                              seg, ntrk, bs, nlayer, kf);
   else if (gp == P2R_GRAD_MIXED)
     launch_adjoint_t<double, float, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt,
-                                               c2_total, ntrk, bs, nlayer, kf);
+                                               c2_total, seg, ntrk, bs, nlayer, kf);
   else
     launch_adjoint_t<double, double, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt,
-                                                c2_total, ntrk, bs, nlayer, kf);
+                                                c2_total, seg, ntrk, bs, nlayer, kf);
+}
+
+// Bytes of the global segment store of one adjoint call (0 when it is in LDS).
+template <class RS, class RL> size_t seg_bytes_rs(int cs, uint64_t ntrk) {
+  if (cs == 4) return SegStore<RS, 4, kRevWaves<RL>>::global_bytes(ntrk);
+  if (cs == 8) return SegStore<RS, 8, kRevWaves<RL>>::global_bytes(ntrk);
+  return 0;
+}
+size_t seg_global_bytes(int gp, int cs, uint64_t ntrk) {
+  if (gp == P2R_GRAD_F32) return seg_bytes_rs<float, float>(cs, ntrk);
+  if (gp == P2R_GRAD_MIXED) return seg_bytes_rs<double, float>(cs, ntrk);
+  return seg_bytes_rs<double, double>(cs, ntrk);
 }
 
 inline size_t ck_bytes_aligned(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -1064,7 +1130,8 @@ struct AdjointPlan {
   uint32_t n = 0, bs = 0;
   int nlayer = 0, nglob = 0, cs = 0, grad_precision = 0;
   unsigned grid = 0;
-  size_t ck_bytes = 0, c2_bytes = 0, ws_bytes = 0, lds = 0;  // workspace: checkpoints | chi2 totals | partials
+  // workspace: checkpoints | chi2 totals | partials | segment store (when not in LDS)
+  size_t ck_bytes = 0, c2_bytes = 0, part_bytes = 0, seg_bytes = 0, ws_bytes = 0, lds = 0;
   p2r_params p{};
 };
 
@@ -1105,8 +1172,10 @@ int adjoint_plan(p2r_ctx* c, const void* trk_in, const void* hits, const void* t
   const size_t nck = (size_t)((pl.nlayer + cs - 1) / cs - 1);
   pl.ck_bytes = std::max<size_t>(nck, 1) * 27 * ntrk * sz;
   pl.c2_bytes = ntrk * sz;
-  const size_t part_bytes = global ? (size_t)pl.grid * pl.nglob * sizeof(double) : 0;
-  pl.ws_bytes = ck_bytes_aligned(pl.ck_bytes) + ck_bytes_aligned(pl.c2_bytes) + part_bytes + 256;
+  pl.part_bytes = global ? (size_t)pl.grid * pl.nglob * sizeof(double) : 0;
+  pl.seg_bytes = seg_global_bytes(grad_precision, cs, ntrk);
+  pl.ws_bytes = ck_bytes_aligned(pl.ck_bytes) + ck_bytes_aligned(pl.c2_bytes) + ck_bytes_aligned(pl.part_bytes) +
+                pl.seg_bytes + 256;
   pl.lds = global ? (size_t)(kAdjThreads / 64) * pl.nglob * sizeof(double) : 0;
   return P2R_OK;
 }
@@ -1117,8 +1186,9 @@ int adjoint_enqueue(p2r_ctx* c, const AdjointPlan& pl, const void* trk_in, const
                     float* chi2, float* grad_par, float* grad_hits, float* grad_kfact, double* grad_global, char* ws,
                     hipStream_t st) {
   char* c2_total = ws + ck_bytes_aligned(pl.ck_bytes);
-  double* partials =
-      grad_global ? reinterpret_cast<double*>(c2_total + ck_bytes_aligned(pl.c2_bytes)) : nullptr;
+  char* part = c2_total + ck_bytes_aligned(pl.c2_bytes);
+  double* partials = grad_global ? reinterpret_cast<double*>(part) : nullptr;
+  char* seg = pl.seg_bytes ? part + ck_bytes_aligned(pl.part_bytes) : nullptr;
   const float kf = pl.p.kfact / pl.p.b_scale;
   auto T = static_cast<const float*>(trk_in);
   auto H = static_cast<const float*>(hits);
@@ -1127,7 +1197,7 @@ int adjoint_enqueue(p2r_ctx* c, const AdjointPlan& pl, const void* trk_in, const
   const bool exact = pl.p.precision == P2R_EXACT;
 #define P2R_ADJ(E, CS)                                                                                         \
   launch_adjoint_p<E, CS>(gp, pl.grid, pl.lds, st, T, H, O, chi2, grad_par, grad_hits, grad_kfact, partials, ws, \
-                          c2_total, pl.n, pl.bs, pl.nlayer, kf)
+                          c2_total, seg, pl.n, pl.bs, pl.nlayer, kf)
   if (exact) {
     if (cs == 1) P2R_ADJ(true, 1); else if (cs == 2) P2R_ADJ(true, 2); else if (cs == 4) P2R_ADJ(true, 4);
     else P2R_ADJ(true, 8);

@@ -29,6 +29,10 @@ using Hit = HitT<float>;
 // loop-invariant only because of reference semantics are recomputed per layer
 // rather than hoisted: each layer then does the full work it is credited with.
 __device__ __forceinline__ void opaque(float& v) { asm volatile("" : "+v"(v)); }
+__device__ __forceinline__ void opaque(double& v) { asm volatile("" : "+v"(v)); }
+// No instruction is scheduled across this point (code size 0): separates phases whose values should not
+// be live at the same time.
+__device__ __forceinline__ void phase_boundary() { __builtin_amdgcn_sched_barrier(0); }
 __device__ __forceinline__ void opaque(Trk& t) {
 #pragma unroll
   for (int i = 0; i < 6; ++i) opaque(t.par[i]);

@@ -4,11 +4,13 @@
 // Instantiates p2r-tests_amd/csrc/p2r_lane_math.inc and p2r_adjoint.inc -- the code the GPU kernels run --
 // with a scalar type that counts every arithmetic operation, and prints for each track read from stdin
 // (27 state floats, the charge, kfact, nlayer, then nlayer hits of 9 floats) and each layer of its chained
-// fit one line "fwd values adjoint cond" of operation totals:
+// fit one line "fwd values adjoint cond iter upd_out" of operation totals:
 //   fwd      propagate_to_r + kalman_update (the forward update; its chi2 included)
-//   values   layer_forward_values (the reverse sweep's recompute of the layer's intermediates)
-//   adjoint  layer_adjoint (that recompute plus the reverse sweep proper)
+//   values   layer_forward_values (the layer's forward values and its updated state)
+//   adjoint  layer_adjoint (the forward values it needs plus the reverse sweep proper)
 //   cond     fixed-point iterations whose convergence branch the reverse took (0..4)
+//   iter     iteration_values of the 5 iterations (what the reverse of the iterations recomputes)
+//   upd_out  update_out (the updated state, which layer_adjoint does not compute)
 // A total counts add/sub, mul, div, sqrt, sin and cos each, and the 4 FP64 operations of the update's
 // determinant and its reciprocal; negation, comparison and selection are not counted.
 #include <cmath>
@@ -42,6 +44,9 @@ inline Cnt& operator-=(Cnt& a, Cnt b) { return a = a - b; }
 inline Cnt& operator*=(Cnt& a, Cnt b) { return a = a * b; }
 inline bool operator<(Cnt a, Cnt b) { return a.v < b.v; }
 inline bool operator>(Cnt a, Cnt b) { return a.v > b.v; }
+// the kernels' optimizer barriers (no code on the GPU either)
+inline void opaque(Cnt&) {}
+inline void phase_boundary() {}
 inline Cnt sqrt(Cnt x) { ++ops(); return Cnt(std::sqrt(x.v)); }
 inline Cnt floor(Cnt x) { return Cnt(std::floor(x.v)); }
 inline void sincos(Cnt x, Cnt* s, Cnt* c) {
@@ -103,7 +108,16 @@ int main() {
       layer_forward_values(in, k, h, f, out);
       const long values = ops() + kDetOps;
       int cond = 0;
-      for (int i = 0; i + 1 < kNiter; ++i) cond += f.cond[i] ? 1 : 0;
+      for (int i = 0; i + 1 < kNiter; ++i) cond += f.x.cond[i] ? 1 : 0;
+      ops() = 0;
+      for (int i = 0; i < kNiter; ++i) {
+        IterFwd it;
+        iteration_values(f.x.xo[i], f.x.yo[i], f.r, in.par[3], f.x.kinv, i + 1 == kNiter, it);
+      }
+      const long iter = ops();
+      ops() = 0;
+      update_out(f.prop, f.u, out);
+      const long upd_out = ops();
       ops() = 0;
       Trk bar;
       for (int i = 0; i < 6; ++i) bar.par[i] = 0.0;
@@ -111,7 +125,7 @@ int main() {
       real kbar = 0.0, hbar[3];
       layer_adjoint(in, k, h, bar, kbar, hbar);
       const long adjoint = ops() + kDetOps;
-      std::printf("%ld %ld %ld %d\n", fwd, values, adjoint, cond);
+      std::printf("%ld %ld %ld %d %ld %ld\n", fwd, values, adjoint, cond, iter, upd_out);
       in = prop;  // the chained fit: the next layer starts from this layer's update
     }
   }

@@ -2,14 +2,17 @@
 
 tests/opcount_harness.cpp instantiates those bodies on the host with a counting scalar type and runs the
 chained fit of tracks from the benchmark generator layer by layer.  Per track-layer it reports the
-operations of the forward update, of the reverse sweep's recompute of the layer's intermediates, and of
-layer_adjoint (that recompute plus the reverse proper), and how many fixed-point iterations had converged
-(the reverse of an iteration's derivative terms runs only then: |r - r0| < 1e-4, tbb.cpp:714).
+operations of the forward update, of the layer's forward values (layer_forward_values), of layer_adjoint,
+how many fixed-point iterations had converged (the reverse of an iteration's derivative terms runs only
+then: |r - r0| < 1e-4, tbb.cpp:714), and the parts that layer_adjoint recomputes or leaves out: each
+iteration's values recomputed from its start point inside the reverse of the iterations
+(iteration_values), and the updated state, which the reverse does not need (update_out).
 
 Frozen here and used by bench.py for the fwd+adjoint roofline:
   forward update               1214  (the reference algorithm's 1212 useful FLOPs, oracle-counted, differ
                                       by the chi2 the product adds and by structural terms it skips)
-  recompute of a layer         1203
+  forward values of a layer    1203  (192 of them the updated state)
+  recomputed in the reverse    84    (each iteration from its start point)
   reverse sweep of a layer     2064 + 246 per converged iteration
   mean reverse on the benchmark's tracks (1024 tracks x 20 layers, chained): 2408 (1.40 converged iterations
   per layer)
@@ -22,6 +25,7 @@ import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REVERSE_BASE, REVERSE_PER_CONVERGED, REVERSE_MEAN = 2064, 246, 2408
+VALUES, ITERATIONS, UPDATE_OUT = 1203, 84, 192
 
 
 @pytest.fixture(scope="module")
@@ -49,12 +53,20 @@ def _counts(exe, ntrk, nlayer=20):
     return np.array([[int(v) for v in l.split()] for l in out.splitlines() if l.strip()])
 
 
+def _reverse(a):
+    """Operations of the reverse proper: layer_adjoint minus the forward values it computes (the layer's
+    values without the updated state, plus each iteration recomputed from its start point)."""
+    fwd, values, adjoint, cond, iters, upd_out = a.T
+    return adjoint - (values - upd_out) - iters
+
+
 def test_adjoint_opcount_frozen(harness):
     a = _counts(harness, 1024)
-    fwd, values, adjoint, cond = a.T
+    fwd, values, adjoint, cond, iters, upd_out = a.T
     assert len(a) == 1024 * 20
-    assert set(fwd) == {1214} and set(values) == {1203}
-    reverse = adjoint - values
+    assert set(fwd) == {1214} and set(values) == {VALUES}
+    assert set(iters) == {ITERATIONS} and set(upd_out) == {UPDATE_OUT}
+    reverse = _reverse(a)
     assert np.array_equal(reverse, REVERSE_BASE + REVERSE_PER_CONVERGED * cond)
     mean = float(reverse.mean())
     print(f"reverse ops per layer: mean {mean:.1f}, converged iterations per layer {cond.mean():.3f}, "
@@ -67,7 +79,7 @@ def test_counter_sees_a_changed_formula(harness):
     frozen base, and each converged iteration adds the same block (so a dropped or extra term changes a
     count above)."""
     a = _counts(harness, 64)
-    rev = a[:, 2] - a[:, 1]
+    rev = _reverse(a)
     for c in np.unique(a[:, 3]):
         assert set(rev[a[:, 3] == c]) == {REVERSE_BASE + REVERSE_PER_CONVERGED * int(c)}
 
