@@ -172,8 +172,9 @@ typedef enum p2r_grad_precision { P2R_GRAD_F32 = 0, P2R_GRAD_MIXED = 1, P2R_GRAD
  * (stride 1 stores every layer's input state and recomputes nothing but the
  * layer's own intermediates).  Workspace: (ceil(nlayer/stride) - 1) * 27 *
  * ntrk states of 4 bytes (P2R_GRAD_F32) or 8 bytes, plus (stride - 2) * 27 *
- * ntrk states for the recomputed layer inputs of a segment where they do not
- * fit in LDS (stride 4 with MIXED or F64, and stride 8).  The context holds one
+ * ntrk values for the recomputed layer inputs of a segment where they do not
+ * fit in LDS (stride 8, and stride 4 with F64), of 4 bytes (F32, MIXED) or 8
+ * (F64).  The context holds one
  * workspace: adjoint calls on one context must not overlap in time (issue
  * them on one stream, or synchronize between streams or threads).
  * trk_out must not overlap trk_in (the reverse sweep rereads the input
@@ -214,7 +215,7 @@ int p2r_forward_adjoint(p2r_ctx* ctx, const void* trk_in, const void* hits, void
  * ceil(ntrk/256) bytes when grad_global is requested, each part rounded up to
  * 256 bytes, plus the segment states of p2r_forward_adjoint's workspace where
  * they are not in LDS, plus 256: at most about 150 000 tracks x 20 layers at
- * stride 4 with P2R_GRAD_F32, about 51 000 with P2R_GRAD_MIXED.
+ * stride 4 with P2R_GRAD_F32, about 76 000 with P2R_GRAD_MIXED.
  * Every call on one GPU shares that GPU's array: calls that overlap in time
  * on one GPU (any stream, thread or context) race and must be serialized by
  * the caller.  Nothing in the array needs clearing between calls. */

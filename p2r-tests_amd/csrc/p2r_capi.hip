@@ -720,18 +720,19 @@ __global__ __launch_bounds__(kAdjThreads, MINW) void adjoint_fwd_kernel(
 
 // Where the reverse sweep keeps the recomputed layer inputs of a segment between its forward recompute and
 // their reverse: the inputs of layers l0+1 .. l0+len-2 (CS-2 states at most; the last one stays in
-// registers, the first is the checkpoint, read again).  In LDS when RW workgroups of them fit beside the
-// global-sum partials, else in a global buffer [slot][field][track] of the workspace.
+// registers, the first is the checkpoint, read again), in the precision RL of the per-layer reverse, which
+// is all the reverse reads of them.  In LDS when RW workgroups of them fit beside the global-sum partials,
+// else in a global buffer [slot][field][track] of the workspace.
 // Register budget of the reverse sweep: 2 waves per SIMD with the FP32 reverse, 1 with the FP64 one.
 template <class RL> constexpr int kRevWaves = sizeof(RL) == sizeof(float) ? 2 : 1;
 constexpr size_t kAdjPartialsLdsMax = (size_t)(kAdjThreads / 64) * (4 * P2R_MAX_NLAYER + 1) * sizeof(double);
-template <class RS, int CS, int RW> struct SegStore {
+template <class RL, int CS, int RW> struct SegStore {
   static constexpr int kSlots = CS > 2 ? CS - 2 : 0;
-  static constexpr size_t kLdsBytes = (size_t)kSlots * 27 * sizeof(RS) * kAdjThreads;
+  static constexpr size_t kLdsBytes = (size_t)kSlots * 27 * sizeof(RL) * kAdjThreads;
   // within 64 KB per workgroup, and RW workgroups (RW waves per SIMD) within the CU's 160 KB
   static constexpr bool kInLds = kSlots == 0 || (kLdsBytes + kAdjPartialsLdsMax <= 64 * 1024 &&
                                                  (size_t)RW * (kLdsBytes + kAdjPartialsLdsMax) <= 160 * 1024);
-  static size_t global_bytes(uint64_t ntrk) { return kInLds ? 0 : (size_t)kSlots * 27 * sizeof(RS) * ntrk; }
+  static size_t global_bytes(uint64_t ntrk) { return kInLds ? 0 : (size_t)kSlots * 27 * sizeof(RL) * ntrk; }
 };
 
 // Reverse sweep of the adjoint, after adjoint_fwd_kernel: per segment, from the last, reload the checkpoint,
@@ -742,13 +743,13 @@ template <class RS, class RL, bool EXACT, int CS, int RW = kRevWaves<RL>>
 __global__ __launch_bounds__(kAdjThreads, RW) void adjoint_kernel(
     const float* __restrict__ trk, const float* __restrict__ hits, const RS* __restrict__ c2_total,
     float* __restrict__ gpar, float* __restrict__ ghit, float* __restrict__ gk, double* __restrict__ partials,
-    const RS* __restrict__ ckpt, RS* __restrict__ seg_global, uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
+    const RS* __restrict__ ckpt, RL* __restrict__ seg_global, uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
   using MS = MathOf<RS, EXACT>;
   // The segment recomputation runs in the requested arithmetic (the layer inputs of the forward pass);
   // the reverse sweep always in the fused instance: gradients have no reference rounding to reproduce,
   // and they are checked against FP64 to a tolerance.
   using ML = MathOf<RL, false>;
-  using SS = SegStore<RS, CS, RW>;
+  using SS = SegStore<RL, CS, RW>;
   constexpr int kWaves = kAdjThreads / 64;
   // dynamic LDS: [kWaves][4*nlayer + 1] doubles when partials != nullptr, then the segment states when in LDS
   extern __shared__ double lds_global[];
@@ -764,19 +765,19 @@ __global__ __launch_bounds__(kAdjThreads, RW) void adjoint_kernel(
   const int nseg = (nlayer + CS - 1) / CS;
   const RS c2 = c2_total[gg];
   const int nglob = 4 * nlayer + 1;
-  RS* seg_lds = reinterpret_cast<RS*>(lds_global + (partials ? kWaves * nglob : 0));
+  RL* seg_lds = reinterpret_cast<RL*>(lds_global + (partials ? kWaves * nglob : 0));
   // slot e of the current lane: LDS [slot][field][thread] or global [slot][field][track]
-  auto seg_at = [&](int slot, int f) -> RS& {
+  auto seg_at = [&](int slot, int f) -> RL& {
     if constexpr (SS::kInLds) return seg_lds[((size_t)slot * 27 + f) * kAdjThreads + threadIdx.x];
     else return seg_global[((size_t)slot * 27 + f) * ntrk + gg];
   };
   auto seg_put = [&](int slot, const TrkT<RS>& t) {
 #pragma unroll
-    for (int f = 0; f < 6; ++f) seg_at(slot, f) = t.par[f];
+    for (int f = 0; f < 6; ++f) seg_at(slot, f) = (RL)t.par[f];
 #pragma unroll
-    for (int f = 0; f < 21; ++f) seg_at(slot, 6 + f) = t.cov[f];
+    for (int f = 0; f < 21; ++f) seg_at(slot, 6 + f) = (RL)t.cov[f];
   };
-  auto seg_get = [&](int slot, TrkT<RS>& t) {
+  auto seg_get = [&](int slot, TrkT<RL>& t) {
 #pragma unroll
     for (int f = 0; f < 6; ++f) t.par[f] = seg_at(slot, f);
 #pragma unroll
@@ -785,7 +786,7 @@ __global__ __launch_bounds__(kAdjThreads, RW) void adjoint_kernel(
   // input of the segment's first layer: the track itself or a checkpoint.  Read again for the segment's first
   // layer: the addresses pass through an opaque copy so that the compiler reloads instead of keeping the
   // values of the first read live through the whole segment.
-  auto seg_start = [&](int s, TrkT<RS>& t) {
+  auto seg_start = [&](int s, auto& t) {  // t: TrkT<RS> or TrkT<RL>
     if (s == 0) {
       const float* tp = tb;
       asm volatile("" : "+v"(tp));
@@ -795,10 +796,12 @@ __global__ __launch_bounds__(kAdjThreads, RW) void adjoint_kernel(
     } else {
       const RS* cp = ckpt + ((size_t)(s - 1) * 27) * ntrk + gg;
       asm volatile("" : "+v"(cp));
+      TrkT<RS> c;
 #pragma unroll
-      for (int f = 0; f < 6; ++f) t.par[f] = cp[(size_t)f * ntrk];
+      for (int f = 0; f < 6; ++f) c.par[f] = cp[(size_t)f * ntrk];
 #pragma unroll
-      for (int f = 0; f < 21; ++f) t.cov[f] = cp[(size_t)(6 + f) * ntrk];
+      for (int f = 0; f < 21; ++f) c.cov[f] = cp[(size_t)(6 + f) * ntrk];
+      convert(c, t);
     }
   };
   // ---- reverse sweep ----
@@ -825,17 +828,15 @@ __global__ __launch_bounds__(kAdjThreads, RW) void adjoint_kernel(
       MS::fwd(cur, (RS)k, h, nxt);
       cur = nxt;
     }
+    // reverse of the segment's layers from the last; each iteration ends by reading the next one's input
+    // (the segment store, then the checkpoint), so that no input is held through another layer's reverse
+    TrkT<RL> in_l;
+    convert(cur, in_l);
 #pragma nounroll
     for (int j = len - 1; j >= 0; --j) {
-      if (j < len - 1) {
-        if (j == 0) seg_start(s, cur);
-        else seg_get(j - 1, cur);
-      }
       const int l = l0 + j;
       HitT<RL> h;
       load_hit_as(hb + l * hstride, bs, h);
-      TrkT<RL> in_l;
-      convert(cur, in_l);
       RL hbar[3];
       ML::adj(in_l, (RL)k, h, bar, kbar, hbar);
       if (active && ghit) {
@@ -855,6 +856,8 @@ __global__ __launch_bounds__(kAdjThreads, RW) void adjoint_kernel(
           lds_global[wave * nglob + 4 * l + 3] = v3;
         }
       }
+      if (j == 1) seg_start(s, in_l);
+      else if (j > 1) seg_get(j - 2, in_l);
     }
   }
   if (active) {
@@ -1071,11 +1074,11 @@ void launch_adjoint_t(unsigned grid, size_t lds, hipStream_t st, const float* T,
   constexpr int kFwdWaves = sizeof(RS) == sizeof(float) ? 3 : 1;
   adjoint_fwd_kernel<RS, EXACT, CS, kFwdWaves><<<grid, kAdjThreads, 0, st>>>(
       T, H, O, chi2, static_cast<RS*>(ckpt), static_cast<RS*>(c2_total), ntrk, bs, nlayer, kf);
-  using SS = SegStore<RS, CS, kRevWaves<RL>>;
+  using SS = SegStore<RL, CS, kRevWaves<RL>>;
   const size_t lds_all = lds + (SS::kInLds ? SS::kLdsBytes : 0);
   adjoint_kernel<RS, RL, EXACT, CS><<<grid, kAdjThreads, lds_all, st>>>(
       T, H, static_cast<const RS*>(c2_total), gpar, ghit, gk, partials, static_cast<const RS*>(ckpt),
-      static_cast<RS*>(seg), ntrk, bs, nlayer, kf);
+      static_cast<RL*>(seg), ntrk, bs, nlayer, kf);
 }
 template <bool EXACT, int CS>
 void launch_adjoint_p(int gp, unsigned grid, size_t lds, hipStream_t st, const float* T, const float* H, float* O,
@@ -1094,8 +1097,8 @@ void launch_adjoint_p(int gp, unsigned grid, size_t lds, hipStream_t st, const f
 
 // Bytes of the global segment store of one adjoint call (0 when it is in LDS).
 template <class RS, class RL> size_t seg_bytes_rs(int cs, uint64_t ntrk) {
-  if (cs == 4) return SegStore<RS, 4, kRevWaves<RL>>::global_bytes(ntrk);
-  if (cs == 8) return SegStore<RS, 8, kRevWaves<RL>>::global_bytes(ntrk);
+  if (cs == 4) return SegStore<RL, 4, kRevWaves<RL>>::global_bytes(ntrk);
+  if (cs == 8) return SegStore<RL, 8, kRevWaves<RL>>::global_bytes(ntrk);
   return 0;
 }
 size_t seg_global_bytes(int gp, int cs, uint64_t ntrk) {

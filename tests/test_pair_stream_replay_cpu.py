@@ -185,13 +185,13 @@ def ragged_500():
     return counts, mask
 
 
-@pytest.mark.parametrize("tile", sorted({TILE, 112, 128}))
+@pytest.mark.parametrize("tile", sorted({TILE, 88, 96, 112, 128}))
 @pytest.mark.parametrize("nlayer", [1, 7, 20, 31, 32])
 @pytest.mark.parametrize("density", [0.0, 0.05, 0.5, 0.95, 1.0])
 @pytest.mark.parametrize("bsize", [1, 8, 1024])
 def test_pair_stream_replay_mask_densities(nlayer, density, bsize, tile):
     """Masks from empty to full, 32-bit masks, tiles cut by the end of the batch and by blocks, at the
-    kernel's tile size and at 112 and 128 slots."""
+    kernel's tile size and at 88, 96, 112 and 128 slots."""
     counts = np.array([40, 17, 0, 33, 300], np.int32)
     offs = _layout(counts, bsize)
     nslot = int(offs[-1]) * bsize

@@ -55,7 +55,7 @@ def _same(a, b):
     return bool(np.array_equal(ua, ub))
 
 
-@pytest.mark.parametrize("gp,cs", [("f32", 4), ("mixed", 2)])
+@pytest.mark.parametrize("gp,cs", [("f32", 4), ("mixed", 2), ("f64", 4)])  # f64 4: segment states in scratch
 def test_noalloc_equals_adjoint_on_two_calls(ctx, gp, cs):
     import p2r_amd
     for shape in (Shape(4, 8192, 20, 32), Shape(1, 2048, 20, 32)):  # the second smaller than the first
