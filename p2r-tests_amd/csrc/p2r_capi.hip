@@ -771,7 +771,11 @@ __global__ __launch_bounds__(kAdjThreads, RW) void adjoint_kernel(
     if constexpr (SS::kInLds) return seg_lds[((size_t)slot * 27 + f) * kAdjThreads + threadIdx.x];
     else return seg_global[((size_t)slot * 27 + f) * ntrk + gg];
   };
+  // Lanes past the batch share the last track's global slots: only the track's own lane writes them (the
+  // others read what it wrote, or stale words; their results are not stored).
+  const bool seg_writer = SS::kInLds || active;
   auto seg_put = [&](int slot, const TrkT<RS>& t) {
+    if (!seg_writer) return;
 #pragma unroll
     for (int f = 0; f < 6; ++f) seg_at(slot, f) = (RL)t.par[f];
 #pragma unroll

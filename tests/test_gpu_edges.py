@@ -107,6 +107,39 @@ def test_adjoint_short_fits_all_strides(ctx, nlayer):
         assert all(np.array_equal(a.view(np.uint8), b.view(np.uint8)) for a, b in zip(r, res[0]))
 
 
+@pytest.mark.parametrize("gp,cs", [("f32", 4), ("f32", 8), ("f64", 4)])
+def test_adjoint_batch_ending_inside_a_workgroup(ctx, gp, cs):
+    """A batch of 1000 tracks (the last 256-lane workgroup 24 lanes short) gives each track the same words
+    as the same tracks inside a batch of 1024.  Covers the segment states in LDS (FP32, stride 4) and in
+    the workspace (FP32 stride 8, FP64 stride 4), where the lanes past the batch share the last track's
+    slots."""
+    import p2r_amd
+    bs, nl = 8, 20
+    big, small = Shape(1, 1024, nl, bs), Shape(1, 1000, nl, bs)
+    trk, hits = p2r_amd.gen_uniform(big)
+    nb = small.nblk
+
+    def run(shape, t, h):
+        dt, dh = torch.from_numpy(np.ascontiguousarray(t)).cuda(), torch.from_numpy(np.ascontiguousarray(h)).cuda()
+        n = shape.ntrk_total
+        bufs = dict(out=torch.empty_like(dt), c2=torch.empty(n, device="cuda"),
+                    gpar=torch.empty(6 * n, device="cuda"), ghit=torch.empty(3 * n * nl, device="cuda"),
+                    gk=torch.empty(n, device="cuda"))
+        ctx.forward_adjoint(dt, dh, bufs["out"], shape, grad_precision=gp, checkpoint_stride=cs, chi2=bufs["c2"],
+                            grad_par=bufs["gpar"], grad_hits=bufs["ghit"], grad_kfact=bufs["gk"], precision="exact")
+        torch.cuda.synchronize()
+        return {k: v.cpu().numpy() for k, v in bufs.items()}
+
+    a = run(big, trk, hits)
+    b = run(small, trk[: nb * 28 * bs], hits[: nb * nl * 9 * bs])
+    n = small.ntrk_total
+    cut = {"out": nb * 28 * bs, "c2": n, "gpar": 6 * n, "ghit": 3 * n * nl, "gk": n}
+    for k, m in cut.items():
+        if k != "out":  # (out holds the charge words, integers)
+            assert np.isfinite(b[k]).all(), k
+        assert _same(b[k], a[k][:m]), (gp, cs, k)
+
+
 def test_empty_batches_are_no_ops(ctx):
     """nevts = 0 (and an uneven batch of empty events) through every entry point: status OK, buffers
     untouched."""
