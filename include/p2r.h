@@ -159,7 +159,9 @@ int p2r_forward_ragged(p2r_ctx* ctx, const void* trk_in, const void* hits, void*
  * the reverse sweep uses fused multiply-adds whatever that precision.
  * P2R_GRAD_F32:   every layer in FP32.
  * P2R_GRAD_MIXED: the state carried from layer to layer (and the checkpoints)
- *                 in FP64, each layer's reverse sweep in FP32.
+ *                 in FP64, each layer's reverse sweep in FP32 but for the
+ *                 first two layers' (FP64), where the fit is least constrained
+ *                 and the covariance update cancels most.
  * P2R_GRAD_F64:   everything in FP64. */
 typedef enum p2r_grad_precision { P2R_GRAD_F32 = 0, P2R_GRAD_MIXED = 1, P2R_GRAD_F64 = 2 } p2r_grad_precision;
 
@@ -215,7 +217,7 @@ int p2r_forward_adjoint(p2r_ctx* ctx, const void* trk_in, const void* hits, void
  * ceil(ntrk/256) bytes when grad_global is requested, each part rounded up to
  * 256 bytes, plus the segment states of p2r_forward_adjoint's workspace where
  * they are not in LDS, plus 256: at most about 150 000 tracks x 20 layers at
- * stride 4 with P2R_GRAD_F32, about 76 000 with P2R_GRAD_MIXED.
+ * stride 4 with P2R_GRAD_F32, about 68 000 with P2R_GRAD_MIXED.
  * Every call on one GPU shares that GPU's array: calls that overlap in time
  * on one GPU (any stream, thread or context) race and must be serialized by
  * the caller.  Nothing in the array needs clearing between calls. */

@@ -667,6 +667,32 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// One wavefront's contributions of layer l to the global sums -- rigid translation dx, dy, dz and
+// rotation dphi of the layer's hits, from the per-hit gradient hbar -- written by lane 0 into the
+// wavefront's row of `lds` ([wave][4*nlayer + 1]).  Tracks out of the sums contribute 0.
+template <class R>
+__device__ __forceinline__ void layer_global_sums(double* lds_row, int l, bool in_global, const R hbar[3], double hx,
+                                                  double hy, int wl) {
+  const double a = in_global ? (double)hbar[0] : 0.0, b = in_global ? (double)hbar[1] : 0.0,
+               cz = in_global ? (double)hbar[2] : 0.0;
+  const double v0 = wave_sum(a), v1 = wave_sum(b), v2 = wave_sum(cz), v3 = wave_sum(-hy * a + hx * b);
+  if (wl == 0) {
+    lds_row[4 * l + 0] = v0;
+    lds_row[4 * l + 1] = v1;
+    lds_row[4 * l + 2] = v2;
+    lds_row[4 * l + 3] = v3;
+  }
+}
+// The workgroup's sums of global entries [j0, j1) over its wavefronts' rows, into partials[j][workgroup].
+__device__ __forceinline__ void write_partials(const double* lds, int nglob, int j0, int j1, double* partials) {
+  __syncthreads();
+  for (int j = j0 + (int)threadIdx.x; j < j1; j += (int)blockDim.x) {
+    double acc = 0.0;
+    for (int w = 0; w < (int)blockDim.x / 64; ++w) acc += lds[w * nglob + j];
+    partials[(size_t)j * gridDim.x + blockIdx.x] = acc;  // [global entry][workgroup]
+  }
+}
+
 template <class R> __device__ __forceinline__ void load_hit_as(const float* hb, uint32_t bs, HitT<R>& h) {
   Hit hf;
   load_hit(hb, bs, hf);
@@ -743,7 +769,8 @@ template <class RS, class RL, bool EXACT, int CS, int RW = kRevWaves<RL>>
 __global__ __launch_bounds__(kAdjThreads, RW) void adjoint_kernel(
     const float* __restrict__ trk, const float* __restrict__ hits, const RS* __restrict__ c2_total,
     float* __restrict__ gpar, float* __restrict__ ghit, float* __restrict__ gk, double* __restrict__ partials,
-    const RS* __restrict__ ckpt, RL* __restrict__ seg_global, uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
+    const RS* __restrict__ ckpt, RL* __restrict__ seg_global, uint32_t ntrk, uint32_t bs, int nlayer, float kf,
+    int l_stop, float* __restrict__ bar_out) {
   using MS = MathOf<RS, EXACT>;
   // The segment recomputation runs in the requested arithmetic (the layer inputs of the forward pass);
   // the reverse sweep always in the fused instance: gradients have no reference rounding to reproduce,
@@ -820,6 +847,7 @@ __global__ __launch_bounds__(kAdjThreads, RW) void adjoint_kernel(
   RL kbar = 0;
   for (int s = nseg - 1; s >= 0; --s) {
     const int l0 = s * CS, len = min(CS, nlayer - l0);
+    if (l0 + len <= l_stop) break;  // the layers below l_stop are left to adjoint_tail_kernel
     TrkT<RS> cur;
     seg_start(s, cur);
     // inputs of layers l0+1 .. l0+len-1: the last stays in `cur`, the others go to the segment store
@@ -837,7 +865,7 @@ __global__ __launch_bounds__(kAdjThreads, RW) void adjoint_kernel(
     TrkT<RL> in_l;
     convert(cur, in_l);
 #pragma nounroll
-    for (int j = len - 1; j >= 0; --j) {
+    for (int j = len - 1; j >= 0 && l0 + j >= l_stop; --j) {
       const int l = l0 + j;
       HitT<RL> h;
       load_hit_as(hb + l * hstride, bs, h);
@@ -848,21 +876,107 @@ __global__ __launch_bounds__(kAdjThreads, RW) void adjoint_kernel(
 #pragma unroll
         for (int c = 0; c < 3; ++c) gh[(size_t)c * bs] = (float)hbar[c];
       }
-      if (partials) {
-        const double hx = h.pos[0], hy = h.pos[1];
-        const double a = in_global ? (double)hbar[0] : 0.0, b = in_global ? (double)hbar[1] : 0.0,
-                     cz = in_global ? (double)hbar[2] : 0.0;
-        const double v0 = wave_sum(a), v1 = wave_sum(b), v2 = wave_sum(cz), v3 = wave_sum(-hy * a + hx * b);
-        if (wl == 0) {
-          lds_global[wave * nglob + 4 * l + 0] = v0;
-          lds_global[wave * nglob + 4 * l + 1] = v1;
-          lds_global[wave * nglob + 4 * l + 2] = v2;
-          lds_global[wave * nglob + 4 * l + 3] = v3;
-        }
-      }
+      if (partials) layer_global_sums(lds_global + wave * nglob, l, in_global, hbar, h.pos[0], h.pos[1], wl);
       if (j == 1) seg_start(s, in_l);
       else if (j > 1) seg_get(j - 2, in_l);
     }
+  }
+  if (active) {
+    if (l_stop > 0) {
+      // handed to adjoint_tail_kernel: the adjoint of layer l_stop's input state, and d/dk so far
+      float* bo = bar_out + g;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) bo[(size_t)f * ntrk] = (float)bar.par[f];
+#pragma unroll
+      for (int f = 0; f < 21; ++f) bo[(size_t)(6 + f) * ntrk] = (float)bar.cov[f];
+      bo[(size_t)27 * ntrk] = (float)kbar;
+    } else {
+      if (gpar) {
+        float* gp = gpar + (size_t)blk * 6 * bs + lane;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) gp[(size_t)f * bs] = (float)bar.par[f];
+      }
+      if (gk) gk[g] = (float)((RL)q * kbar);  // d/dkfact = q d/dk
+    }
+  }
+  if (partials) {
+    if (l_stop == 0) {
+      // b_scale: kfact_eff = kfact / b  =>  d/db = -kfact_eff * d/dkfact_eff at b = 1 scaled by 1/b; the
+      // host passes kf = kfact/b, so d/db = -(kf / b) q kbar; written here for b folded into kf (b = 1 form)
+      const double vb = wave_sum(in_global ? -(double)kf * (double)q * (double)kbar : 0.0);
+      if (wl == 0) lds_global[wave * nglob + 4 * nlayer] = vb;
+    }
+    // this kernel's entries: its layers, and the field scale when no tail follows
+    write_partials(lds_global, nglob, 4 * l_stop, l_stop > 0 ? 4 * nlayer : nglob, partials);
+  }
+}
+
+// The first layers of a MIXED adjoint in FP64.  With the state in FP64 the per-layer FP32 reverse loses
+// accuracy mostly at the first layers, where the fit is least constrained and the covariance update
+// cancels most: adjoint_kernel stops at layer l_stop (<= kTailLayers) and hands over the adjoint of that
+// layer's input; this kernel recomputes the inputs of layers 0 .. l_stop-1 in the requested arithmetic
+// (FP64 states, as in the forward pass), back-propagates them in FP64 and writes the per-track outputs,
+// those layers' global-sum partials and the field scale's.  Its own kernel so that the FP64 reverse's
+// registers (1 wavefront per SIMD) do not set the occupancy of the FP32 sweep.
+constexpr int kTailLayers = 2;
+template <bool EXACT>
+__global__ __launch_bounds__(kAdjThreads) void adjoint_tail_kernel(
+    const float* __restrict__ trk, const float* __restrict__ hits, const double* __restrict__ c2_total,
+    const float* __restrict__ bar_in, float* __restrict__ gpar, float* __restrict__ ghit, float* __restrict__ gk,
+    double* __restrict__ partials, uint32_t ntrk, uint32_t bs, int nlayer, float kf, int l_stop) {
+  using MS = MathOf<double, EXACT>;
+  using ML = MathOf<double, false>;
+  extern __shared__ double lds_global[];  // [kWaves][4*nlayer + 1] when partials != nullptr
+  const uint32_t g = blockIdx.x * kAdjThreads + threadIdx.x;
+  const bool active = g < ntrk;
+  const uint32_t gg = active ? g : ntrk - 1;  // lanes past the batch recompute the last track, write nothing
+  const uint32_t blk = gg / bs, lane = gg - blk * bs;
+  const float* tb = trk + (size_t)blk * 28 * bs + lane;
+  const float* hb = hits + (size_t)blk * nlayer * 9 * bs + lane;
+  const size_t hstride = (size_t)9 * bs;
+  const int q = reinterpret_cast<const int*>(tb)[(size_t)27 * bs];
+  const float k = (float)q * kf;
+  const bool in_global = active && isfinite(c2_total[gg]);
+  const int nglob = 4 * nlayer + 1;
+  const int wave = threadIdx.x >> 6, wl = threadIdx.x & 63;
+  // inputs of layers 0 .. l_stop-1: the track, then the fit in FP64
+  TrkT<double> st[kTailLayers];
+  {
+    Trk in_f;
+    load_trk(tb, bs, in_f);
+    convert(in_f, st[0]);
+  }
+#pragma unroll
+  for (int j = 1; j < kTailLayers; ++j) {
+    if (j < l_stop) {
+      HitT<double> h;
+      load_hit_as(hb + (j - 1) * hstride, bs, h);
+      MS::fwd(st[j - 1], (double)k, h, st[j]);
+    }
+  }
+  TrkT<double> bar;
+  double kbar;
+  {
+    const float* bi = bar_in + gg;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) bar.par[f] = bi[(size_t)f * ntrk];
+#pragma unroll
+    for (int f = 0; f < 21; ++f) bar.cov[f] = bi[(size_t)(6 + f) * ntrk];
+    kbar = bi[(size_t)27 * ntrk];
+  }
+#pragma unroll
+  for (int l = kTailLayers - 1; l >= 0; --l) {
+    if (l >= l_stop) continue;
+    HitT<double> h;
+    load_hit_as(hb + l * hstride, bs, h);
+    double hbar[3];
+    ML::adj(st[l], (double)k, h, bar, kbar, hbar);
+    if (active && ghit) {
+      float* gh = ghit + ((size_t)blk * nlayer + l) * 3 * bs + lane;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) gh[(size_t)c * bs] = (float)hbar[c];
+    }
+    if (partials) layer_global_sums(lds_global + wave * nglob, l, in_global, hbar, h.pos[0], h.pos[1], wl);
   }
   if (active) {
     if (gpar) {
@@ -870,18 +984,17 @@ __global__ __launch_bounds__(kAdjThreads, RW) void adjoint_kernel(
 #pragma unroll
       for (int f = 0; f < 6; ++f) gp[(size_t)f * bs] = (float)bar.par[f];
     }
-    if (gk) gk[g] = (float)((RL)q * kbar);  // d/dkfact = q d/dk
+    if (gk) gk[g] = (float)((double)q * kbar);  // d/dkfact = q d/dk
   }
   if (partials) {
-    // b_scale: kfact_eff = kfact / b  =>  d/db = -kfact_eff * d/dkfact_eff at b = 1 scaled by 1/b; the
-    // host passes kf = kfact/b, so d/db = -(kf / b) q kbar; written here for b folded into kf (b = 1 form)
-    const double vb = wave_sum(in_global ? -(double)kf * (double)q * (double)kbar : 0.0);
+    // the field scale, as in adjoint_kernel
+    const double vb = wave_sum(in_global ? -(double)kf * (double)q * kbar : 0.0);
     if (wl == 0) lds_global[wave * nglob + 4 * nlayer] = vb;
-    __syncthreads();
-    for (int j = threadIdx.x; j < nglob; j += kAdjThreads) {
+    write_partials(lds_global, nglob, 0, 4 * l_stop, partials);
+    if (threadIdx.x == 0) {  // the field-scale entry, summed over the wavefronts like the others
       double acc = 0.0;
-      for (int w = 0; w < kWaves; ++w) acc += lds_global[w * nglob + j];
-      partials[(size_t)j * gridDim.x + blockIdx.x] = acc;  // [global entry][workgroup]
+      for (int w = 0; w < kAdjThreads / 64; ++w) acc += lds_global[w * nglob + 4 * nlayer];
+      partials[(size_t)(4 * nlayer) * gridDim.x + blockIdx.x] = acc;
     }
   }
 }
@@ -1073,30 +1186,36 @@ int launch_ragged_s(bool chi2_on, bool exact, int rounds, p2r_ctx* c, hipStream_
 template <class RS, class RL, bool EXACT, int CS>
 void launch_adjoint_t(unsigned grid, size_t lds, hipStream_t st, const float* T, const float* H, float* O, float* chi2,
                       float* gpar, float* ghit, float* gk, double* partials, void* ckpt, void* c2_total, void* seg,
-                      uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
+                      float* bar, uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
   // the forward pass at the forward kernels' register budget (3 waves per SIMD in FP32)
   constexpr int kFwdWaves = sizeof(RS) == sizeof(float) ? 3 : 1;
   adjoint_fwd_kernel<RS, EXACT, CS, kFwdWaves><<<grid, kAdjThreads, 0, st>>>(
       T, H, O, chi2, static_cast<RS*>(ckpt), static_cast<RS*>(c2_total), ntrk, bs, nlayer, kf);
   using SS = SegStore<RL, CS, kRevWaves<RL>>;
   const size_t lds_all = lds + (SS::kInLds ? SS::kLdsBytes : 0);
+  // MIXED (FP64 state, FP32 reverse): the first layers' reverse in FP64, in adjoint_tail_kernel
+  constexpr bool kTail = sizeof(RS) == sizeof(double) && sizeof(RL) == sizeof(float);
+  const int l_stop = kTail ? std::min(kTailLayers, nlayer) : 0;
   adjoint_kernel<RS, RL, EXACT, CS><<<grid, kAdjThreads, lds_all, st>>>(
       T, H, static_cast<const RS*>(c2_total), gpar, ghit, gk, partials, static_cast<const RS*>(ckpt),
-      static_cast<RL*>(seg), ntrk, bs, nlayer, kf);
+      static_cast<RL*>(seg), ntrk, bs, nlayer, kf, l_stop, bar);
+  if (kTail)
+    adjoint_tail_kernel<EXACT><<<grid, kAdjThreads, lds, st>>>(T, H, static_cast<const double*>(c2_total), bar, gpar,
+                                                               ghit, gk, partials, ntrk, bs, nlayer, kf, l_stop);
 }
 template <bool EXACT, int CS>
 void launch_adjoint_p(int gp, unsigned grid, size_t lds, hipStream_t st, const float* T, const float* H, float* O,
                       float* chi2, float* gpar, float* ghit, float* gk, double* partials, void* ckpt, void* c2_total,
-                      void* seg, uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
+                      void* seg, float* bar, uint32_t ntrk, uint32_t bs, int nlayer, float kf) {
   if (gp == P2R_GRAD_F32)
     launch_adjoint_t<float, float, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt, c2_total,
-                                              seg, ntrk, bs, nlayer, kf);
+                                              seg, bar, ntrk, bs, nlayer, kf);
   else if (gp == P2R_GRAD_MIXED)
     launch_adjoint_t<double, float, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt,
-                                               c2_total, seg, ntrk, bs, nlayer, kf);
+                                               c2_total, seg, bar, ntrk, bs, nlayer, kf);
   else
     launch_adjoint_t<double, double, EXACT, CS>(grid, lds, st, T, H, O, chi2, gpar, ghit, gk, partials, ckpt,
-                                                c2_total, seg, ntrk, bs, nlayer, kf);
+                                                c2_total, seg, bar, ntrk, bs, nlayer, kf);
 }
 
 // Bytes of the global segment store of one adjoint call (0 when it is in LDS).
@@ -1137,8 +1256,9 @@ struct AdjointPlan {
   uint32_t n = 0, bs = 0;
   int nlayer = 0, nglob = 0, cs = 0, grad_precision = 0;
   unsigned grid = 0;
-  // workspace: checkpoints | chi2 totals | partials | segment store (when not in LDS)
-  size_t ck_bytes = 0, c2_bytes = 0, part_bytes = 0, seg_bytes = 0, ws_bytes = 0, lds = 0;
+  // workspace: checkpoints | chi2 totals | partials | segment store (when not in LDS) | the MIXED tail's
+  // hand-over (the adjoint state at layer kTailLayers, 28 floats per track)
+  size_t ck_bytes = 0, c2_bytes = 0, part_bytes = 0, seg_bytes = 0, bar_bytes = 0, ws_bytes = 0, lds = 0;
   p2r_params p{};
 };
 
@@ -1181,8 +1301,9 @@ int adjoint_plan(p2r_ctx* c, const void* trk_in, const void* hits, const void* t
   pl.c2_bytes = ntrk * sz;
   pl.part_bytes = global ? (size_t)pl.grid * pl.nglob * sizeof(double) : 0;
   pl.seg_bytes = seg_global_bytes(grad_precision, cs, ntrk);
+  pl.bar_bytes = grad_precision == P2R_GRAD_MIXED ? (size_t)28 * sizeof(float) * ntrk : 0;
   pl.ws_bytes = ck_bytes_aligned(pl.ck_bytes) + ck_bytes_aligned(pl.c2_bytes) + ck_bytes_aligned(pl.part_bytes) +
-                pl.seg_bytes + 256;
+                ck_bytes_aligned(pl.seg_bytes) + pl.bar_bytes + 256;
   pl.lds = global ? (size_t)(kAdjThreads / 64) * pl.nglob * sizeof(double) : 0;
   return P2R_OK;
 }
@@ -1196,6 +1317,9 @@ int adjoint_enqueue(p2r_ctx* c, const AdjointPlan& pl, const void* trk_in, const
   char* part = c2_total + ck_bytes_aligned(pl.c2_bytes);
   double* partials = grad_global ? reinterpret_cast<double*>(part) : nullptr;
   char* seg = pl.seg_bytes ? part + ck_bytes_aligned(pl.part_bytes) : nullptr;
+  float* bar = pl.bar_bytes
+                   ? reinterpret_cast<float*>(part + ck_bytes_aligned(pl.part_bytes) + ck_bytes_aligned(pl.seg_bytes))
+                   : nullptr;
   const float kf = pl.p.kfact / pl.p.b_scale;
   auto T = static_cast<const float*>(trk_in);
   auto H = static_cast<const float*>(hits);
@@ -1204,7 +1328,7 @@ int adjoint_enqueue(p2r_ctx* c, const AdjointPlan& pl, const void* trk_in, const
   const bool exact = pl.p.precision == P2R_EXACT;
 #define P2R_ADJ(E, CS)                                                                                         \
   launch_adjoint_p<E, CS>(gp, pl.grid, pl.lds, st, T, H, O, chi2, grad_par, grad_hits, grad_kfact, partials, ws, \
-                          c2_total, seg, pl.n, pl.bs, pl.nlayer, kf)
+                          c2_total, seg, bar, pl.n, pl.bs, pl.nlayer, kf)
   if (exact) {
     if (cs == 1) P2R_ADJ(true, 1); else if (cs == 2) P2R_ADJ(true, 2); else if (cs == 4) P2R_ADJ(true, 4);
     else P2R_ADJ(true, 8);

@@ -115,9 +115,10 @@ def test_adjoint_f64_matches_dual_and_fd(orc, ctx, batch):
 
 @pytest.mark.parametrize("gp,median_bound,max_bound", [("mixed", 1e-3, 1e-3), ("f32", 1.5e-3, 1.5e-2)])
 def test_adjoint_fp32_gradients_vs_fp64(orc, ctx, batch, gp, median_bound, max_bound):
-    """FP32 arithmetic against FP64 FD on 64 tracks.  Carrying the state in FP64 (mixed, the default gradient
-    precision) keeps every track within 1e-3; the chained fit's all-FP32 states carry the Kalman update's
-    cancellation, so the all-FP32 gradient is bounded per track only to ~1e-2."""
+    """FP32 arithmetic against FP64 FD on 64 tracks.  Carrying the state in FP64, with the first two
+    layers' reverse in FP64 (mixed, the default gradient precision), keeps every track within 1e-3; the
+    chained fit's all-FP32 states carry the Kalman update's cancellation, so the all-FP32 gradient is
+    bounded per track only to ~1e-2."""
     _, _, grads, _ = run_adjoint(ctx, *batch[1:], batch[0], gp)
     e = _errors(orc, grads, batch, "fd")
     print(f"{gp} adjoint vs FP64 FD on 64 tracks: median {np.median(e):.2e} p90 {np.quantile(e, 0.9):.2e} "
@@ -133,12 +134,14 @@ def test_checkpoint_stride_does_not_change_gradients(ctx, batch):
         assert np.array_equal(g.view(np.uint64), ref.view(np.uint64)), cs
 
 
-def test_global_gradients_reduce_the_per_hit_gradients(ctx, batch):
+@pytest.mark.parametrize("gp", ["f64", "mixed"])
+def test_global_gradients_reduce_the_per_hit_gradients(ctx, batch, gp):
     """grad_global = sums over tracks of the per-hit gradients (dx, dy, dz, dphi per layer) and of
-    -kfact * d/dkfact (field scale), reduced in FP64 in a fixed order: bit-reproducible run to run."""
+    -kfact * d/dkfact (field scale), reduced in FP64 in a fixed order: bit-reproducible run to run.  With
+    mixed, the FP64 reverse of the first two layers writes their entries and the field scale's."""
     shape, trk, hits = batch
-    _, _, grads, gg = run_adjoint(ctx, trk, hits, shape, "f64", with_global=True)
-    _, _, _, gg2 = run_adjoint(ctx, trk, hits, shape, "f64", with_global=True)
+    _, _, grads, gg = run_adjoint(ctx, trk, hits, shape, gp, with_global=True)
+    _, _, _, gg2 = run_adjoint(ctx, trk, hits, shape, gp, with_global=True)
     assert np.array_equal(gg.view(np.uint64), gg2.view(np.uint64))
     L, bs = shape.nlayer, shape.bsize
     H = hits.reshape(-1, L, 9, bs).transpose(0, 3, 1, 2).reshape(-1, L, 9).astype(np.float64)

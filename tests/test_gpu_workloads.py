@@ -166,14 +166,13 @@ def test_fit_100x9600x20_stride4_outputs_match_oracle(orc, ctx, fit_100):
     assert n_state == 0 and n_chi2 == 0
 
 
-@pytest.mark.parametrize("gp,max_bound,median_bound", [("mixed", 2e-3, 1e-3), ("f64", 1e-5, 1e-6),
+@pytest.mark.parametrize("gp,max_bound,median_bound", [("mixed", 1e-3, 5e-4), ("f64", 1e-5, 1e-6),
                                                        ("f32", 1.5e-2, 1.5e-3)])
 def test_fit_100x9600x20_stride4_gradients(ctx, fit_100, fit_100_refs, gp, max_bound, median_bound):
     """FP64 within 1e-5 of FP64 central differences (their own truncation) on every sampled track.  The
-    default gradient precision (mixed: FP64 state, FP32 reverse) has its median near 3e-4 and stays within
-    1e-3 on all but about one track in 500 (1.25e-3 on one of these 64: its FP32 recompute of the second
-    layer); all-FP32 carries the FP32 update's cancellation in its fitted states (a quarter of the tracks
-    above 1e-3)."""
+    default gradient precision (mixed: FP64 state, FP32 reverse but for the first two layers') within 1e-3
+    on every sampled track, the north star's bar; all-FP32 carries the FP32 update's cancellation in its
+    fitted states (a quarter of the tracks above 1e-3)."""
     shape, trk, hits, tracks = fit_100
     dual, fd = fit_100_refs
     _say(f"fit + adjoint {shape}, stride 4, {gp}: 64 random tracks {tracks[:4]}... against Dual and FD")
