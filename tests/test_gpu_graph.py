@@ -78,7 +78,7 @@ def test_graph_ragged_replays_match_eager(ctx, rounds):
     g.close()
 
 
-@pytest.mark.parametrize("gp", ["f32", "f64"])
+@pytest.mark.parametrize("gp", ["f32", "mixed", "f64"])  # mixed: a third kernel (the FP64 first layers)
 def test_graph_adjoint_replays_match_eager(ctx, gp):
     import p2r_amd
     shape = Shape(2, 1024, 20, 32)
