@@ -33,10 +33,10 @@ for _p in (ROOT, os.path.join(ROOT, "p2r-tests_amd"), os.path.join(ROOT, "tests"
 # Jacobian; the roofline credits the 1212 non-trivial ones.
 FLOP_PER_UPDATE = 1212
 # FLOPs of the reverse sweep of one track-layer (the adjoint of the update, its chi2 and the propagation),
-# counted on the product's own per-lane code by tests/opcount_harness.cpp: 1878 plus 246 for each fixed-point
-# iteration whose convergence branch is taken, 2222 on average over the benchmark's tracks.  The reverse's
+# counted on the product's own per-lane code by tests/opcount_harness.cpp: 1878 plus 164 for each fixed-point
+# iteration whose convergence branch is taken, 2107 on average over the benchmark's tracks.  The reverse's
 # recompute of the forward values (checkpointing) is implementation work and not credited.
-REVERSE_FLOP_PER_UPDATE = 2222
+REVERSE_FLOP_PER_UPDATE = 2107
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (AMD published)
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E (AMD published)
 HIT_BYTES = 36             # per track-layer

@@ -13,8 +13,8 @@ Frozen here and used by bench.py for the fwd+adjoint roofline:
                                       by the chi2 the product adds and by structural terms it skips)
   forward values of a layer    1203  (192 of them the updated state)
   recomputed in the reverse    84    (each iteration from its start point)
-  reverse sweep of a layer     1878 + 246 per converged iteration
-  mean reverse on the benchmark's tracks (1024 tracks x 20 layers, chained): 2222 (1.40 converged iterations
+  reverse sweep of a layer     1878 + 164 per converged iteration
+  mean reverse on the benchmark's tracks (1024 tracks x 20 layers, chained): 2107 (1.40 converged iterations
   per layer)
 """
 import os
@@ -24,7 +24,7 @@ import numpy as np
 import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-REVERSE_BASE, REVERSE_PER_CONVERGED, REVERSE_MEAN = 1878, 246, 2222
+REVERSE_BASE, REVERSE_PER_CONVERGED, REVERSE_MEAN = 1878, 164, 2107
 VALUES, ITERATIONS, UPDATE_OUT = 1203, 84, 192
 
 
