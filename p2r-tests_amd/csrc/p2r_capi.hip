@@ -112,12 +112,13 @@ struct WaveHits {
 // MINW: minimum waves per SIMD the register allocation must allow (1 = the compiler's choice; 3 or 4
 // trade spills for occupancy, a tuning choice).
 // In reference semantics every layer restarts from the input track, whose covariance the propagation
-// reads only in the transport at its end: it waits in LDS (one column of [21][blockDim.x] per lane)
-// instead of holding 21 registers through the helix iterations.  The parameters stay in registers.
+// reads only in the transport at its end: it waits in LDS (21 consecutive words per lane, [blockDim.x][21])
+// instead of holding 21 registers through the helix iterations.  The parameters stay in registers.  With
+// the lane's words consecutive, one base address serves all 21 reads (the fields are immediate offsets, read
+// in pairs), and the odd stride of 21 words puts consecutive lanes in distinct banks.
 struct LdsCov {
-  const float* lds;
-  uint32_t col, stride;
-  __device__ __forceinline__ float operator[](int i) const { return lds[(uint32_t)i * stride + col]; }
+  const float* row;  // the lane's 21 words
+  __device__ __forceinline__ float operator[](int i) const { return row[i]; }
 };
 
 template <int SEM, bool CHI2, bool EXACT, int MINW>
@@ -125,10 +126,10 @@ __global__ __launch_bounds__(kThreads, MINW) void fwd_uniform_kernel(const float
                                                                float* out, float* __restrict__ chi2, uint32_t ntrk,
                                                                uint32_t bs, int nlayer, float kfact, int l0, int l1,
                                                                bool acc, bool store, int opts) {
-  extern __shared__ float cov_lds[];  // [21][blockDim.x], reference semantics only
+  extern __shared__ float cov_lds[];  // [blockDim.x][21], reference semantics only
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ntrk) return;
-  const uint32_t blk = g / bs, lane = g - blk * bs, stride = blockDim.x;
+  const uint32_t blk = g / bs, lane = g - blk * bs;
   const float* tb = trk + (size_t)blk * 28 * bs + lane;
   Trk cur;  // chained: the state between layers
   float in_par[6];
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(kThreads, MINW) void fwd_uniform_kernel(const float
 #pragma unroll
     for (int f = 0; f < 6; ++f) in_par[f] = tb[(size_t)f * bs];
 #pragma unroll
-    for (int f = 0; f < 21; ++f) cov_lds[f * stride + threadIdx.x] = tb[(size_t)(6 + f) * bs];
+    for (int f = 0; f < 21; ++f) cov_lds[threadIdx.x * 21u + f] = tb[(size_t)(6 + f) * bs];
   } else {
     load_trk(tb, bs, cur);
   }
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(kThreads, MINW) void fwd_uniform_kernel(const float
       asm volatile("" : "+v"(col));
       q = q_in;
       k = (float)q * kfact;
-      Lane<EXACT>::propagate(in_par, LdsCov{cov_lds, col, stride}, k, h.pos[0], h.pos[1], prop);
+      Lane<EXACT>::propagate(in_par, LdsCov{cov_lds + col * 21u}, k, h.pos[0], h.pos[1], prop);
     } else {
       Lane<EXACT>::propagate(cur, k, h.pos[0], h.pos[1], prop);  // propagate-tor-test_hip.cu:842-845
     }
