@@ -83,6 +83,10 @@ using Jac = JacT<real>;
 #undef P2R_DOUBLE
 }  // namespace fast
 
+// FP64 hardware estimates of 1/x and 1/sqrt(x) (v_rcp_f64, v_rsq_f64), refined by the FP64 instance.
+__device__ __forceinline__ double rcp_est(double x) { return __builtin_amdgcn_rcp(x); }
+__device__ __forceinline__ double rsq_est(double x) { return __builtin_amdgcn_rsq(x); }
+
 namespace dbl {
 #pragma clang fp contract(fast)
 using real = double;
@@ -91,10 +95,12 @@ using Hit = HitT<real>;
 using Jac = JacT<real>;
 #define P2R_EXACT 0
 #define P2R_DOUBLE 1
+#define P2R_DOUBLE_EST 1
 #include "p2r_lane_math.inc"
 #include "p2r_adjoint.inc"
 #undef P2R_EXACT
 #undef P2R_DOUBLE
+#undef P2R_DOUBLE_EST
 }  // namespace dbl
 
 // Compile-time selection of the instance.

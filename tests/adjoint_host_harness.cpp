@@ -2,7 +2,8 @@
 // tests/test_adjoint_host_cpu.py).
 //
 // Instantiates p2r-tests_amd/csrc/p2r_lane_math.inc and p2r_adjoint.inc -- the code the GPU kernels run --
-// with real = double, as the library's FP64 instance does, and runs the chained fit of each track read
+// with real = double, as the library's FP64 instance does (with IEEE division and square root and libm's
+// sin/cos where that instance refines hardware estimates), and runs the chained fit of each track read
 // from stdin (27 state floats, the charge, kfact, nlayer, then nlayer hits of 9 floats) forward with
 // layer_forward_values, then backward with layer_adjoint.  Prints per track one line of the gradient of
 // the total chi2: d/d(6 initial parameters), d/d(hit x, y, z) per layer, d/dkfact -- the layout of the
