@@ -53,6 +53,11 @@ template <class A, class B> __device__ __forceinline__ void convert(const HitT<A
   for (int i = 0; i < 6; ++i) b.cov[i] = (B)a.cov[i];
 }
 
+// FP64 hardware estimates of 1/x and 1/sqrt(x) (v_rcp_f64, v_rsq_f64), refined by Newton steps in the
+// FP64 instance's division and square root and in the fast instance's FP64 reciprocal.
+__device__ __forceinline__ double rcp_est(double x) { return __builtin_amdgcn_rcp(x); }
+__device__ __forceinline__ double rsq_est(double x) { return __builtin_amdgcn_rsq(x); }
+
 // Instances of the per-lane math (see p2r_lane_math.inc) and of its adjoint
 // (p2r_adjoint.inc).
 namespace exact {
@@ -82,10 +87,6 @@ using Jac = JacT<real>;
 #undef P2R_EXACT
 #undef P2R_DOUBLE
 }  // namespace fast
-
-// FP64 hardware estimates of 1/x and 1/sqrt(x) (v_rcp_f64, v_rsq_f64), refined by the FP64 instance.
-__device__ __forceinline__ double rcp_est(double x) { return __builtin_amdgcn_rcp(x); }
-__device__ __forceinline__ double rsq_est(double x) { return __builtin_amdgcn_rsq(x); }
 
 namespace dbl {
 #pragma clang fp contract(fast)
