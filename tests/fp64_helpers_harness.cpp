@@ -6,8 +6,11 @@
 // the exact value times (1 + d): d is drawn per call from [-2^-E, 2^-E] (E from the command line; the
 // hardware's estimates are documented to within 2^29 ulps, about 2^-23, closer than E = 20).  Prints, over
 // N random operands of magnitudes 1e-30 to 1e30 and both signs, the largest error of m_rcp, m_div and m_sqrt
-// in ulps of the IEEE result, then the special values "sqrt0 sqrtinf sqrtneg" (1 where m_sqrt(0) == 0,
-// m_sqrt(inf) == inf, m_sqrt(-1) is NaN).
+// in ulps of the IEEE result; then the special values "sqrt0 sqrtinf sqrtneg" (1 where m_sqrt(0) == 0,
+// m_sqrt(inf) == inf, m_sqrt(-1) is NaN); then, over N arguments uniform in [-1e5, 1e5], N in [-4, 4] and
+// the doubles nearest to k pi/2 for |k| <= 1000, the largest error of m_sincos's sine and cosine against
+// libm's, in units of 2^-52 (absolute error; both lie in [-1, 1]), and "sinnan" (1 where m_sincos of
+// NaN and of infinity gives NaN).
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -81,5 +84,19 @@ int main(int argc, char** argv) {
   std::printf("%.3f %.3f %.3f\n", e_rcp, e_div, e_sqrt);
   std::printf("%d %d %d\n", m_sqrt(0.0) == 0.0 ? 1 : 0, m_sqrt((double)INFINITY) == (double)INFINITY ? 1 : 0,
               std::isnan(m_sqrt(-1.0)) ? 1 : 0);
+  double e_sin = 0, e_cos = 0;
+  auto check = [&](double x) {
+    double sv, cv;
+    m_sincos(x, sv, cv);
+    e_sin = std::fmax(e_sin, std::fabs(sv - std::sin(x)) * 0x1.0p52);
+    e_cos = std::fmax(e_cos, std::fabs(cv - std::cos(x)) * 0x1.0p52);
+  };
+  for (long i = 0; i < n; ++i) check(-1.0e5 + 2.0e5 * uniform());
+  for (long i = 0; i < n; ++i) check(-4.0 + 8.0 * uniform());
+  for (int k = -1000; k <= 1000; ++k) check((double)k * 1.5707963267948966);
+  double sn, cn, si, ci;
+  m_sincos((double)NAN, sn, cn);
+  m_sincos((double)INFINITY, si, ci);
+  std::printf("%.3f %.3f %d\n", e_sin, e_cos, std::isnan(sn) && std::isnan(cn) && std::isnan(si) && std::isnan(ci));
   return 0;
 }
