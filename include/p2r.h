@@ -182,9 +182,14 @@ typedef enum p2r_grad_precision { P2R_GRAD_F32 = 0, P2R_GRAD_MIXED = 1, P2R_GRAD
  * trk_out must not overlap trk_in (the reverse sweep rereads the input
  * tracks after the forward pass has written trk_out): P2R_ERR_ARG.
  * Outputs (device memory; any may be NULL except trk_out):
- *   trk_out     final fitted tracks: with P2R_GRAD_F32 word for word those of
- *               p2r_forward (P2R_CHAINED, same precision); with MIXED and F64
- *               the fit carried in FP64, rounded to float
+ *   trk_out     final fitted tracks: with P2R_GRAD_F32 and P2R_EXACT word for
+ *               word those of p2r_forward (P2R_CHAINED); with P2R_GRAD_F32 and
+ *               P2R_FAST the same fused formulas, which the compiler may fuse
+ *               differently in this kernel (fast-mode accuracy, not p2r_forward's
+ *               bits); with MIXED and F64 the fit carried in FP64, rounded to
+ *               float.  With P2R_EXACT and P2R_GRAD_F32 the gradients are the same
+ *               words at every checkpoint stride; otherwise they agree across
+ *               strides to the arithmetic's precision.
  *   chi2        float[ntrk]: total chi2 per track (flat track index)
  *   grad_par    float, MP6F blocks: d chi2 / d (initial x, y, z, 1/pt, phi, theta)
  *   grad_hits   float, one MP3F block per hit block (index block*nlayer + L):

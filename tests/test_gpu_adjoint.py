@@ -104,6 +104,22 @@ def test_adjoint_forward_outputs_equal_p2r_forward(ctx, batch):
     assert np.array_equal(c2.view(np.uint32), dc.cpu().numpy().view(np.uint32))
 
 
+def test_adjoint_fast_forward_outputs_meet_the_fast_criterion(orc, ctx, batch):
+    """With fast arithmetic the adjoint's forward pass is its own compilation of the fused formulas (the
+    compiler fuses multiply-adds per kernel), so its outputs are held to the fast-mode criterion of the
+    plain forward (tests/parity.py::compare_statistical: as close to the IEEE-strict oracle as the
+    reference's own fast-math build) rather than to p2r_forward's bits."""
+    import parity
+    shape, trk, hits = batch
+    out, _, _, _ = run_adjoint(ctx, trk, hits, shape, "f32", "fast")
+    cpu = orc.run_f32(trk, hits, shape.nblk, shape.nlayer, shape.bsize, semantics=1)
+    alt = oracle.Oracle("liboracle_bench.so").run_f32(trk, hits, shape.nblk, shape.nlayer, shape.bsize, semantics=1)
+    par64, cov64, _ = orc.run_f64(trk, hits, shape.nblk, shape.nlayer, shape.bsize, semantics=1)
+    res = parity.compare_statistical(out.ravel(), cpu, alt, shape.bsize, par64, cov64)
+    print("fast adjoint forward pass", res.get("frac_within_rtol_of_cpu"), res.get("ok"), flush=True)
+    assert res["ok"], res
+
+
 def test_adjoint_f64_matches_dual_and_fd(orc, ctx, batch):
     _, _, grads, _ = run_adjoint(ctx, *batch[1:], batch[0], "f64")
     e_dual = _errors(orc, grads, batch, "dual")
@@ -113,25 +129,46 @@ def test_adjoint_f64_matches_dual_and_fd(orc, ctx, batch):
     assert e_fd.max() < 1e-4
 
 
-@pytest.mark.parametrize("gp,median_bound,max_bound", [("mixed", 1e-3, 1e-3), ("f32", 1.5e-3, 1.5e-2)])
-def test_adjoint_fp32_gradients_vs_fp64(orc, ctx, batch, gp, median_bound, max_bound):
+@pytest.mark.parametrize("gp,precision,median_bound,max_bound", [("mixed", "exact", 1e-3, 1e-3),
+                                                                  ("f32", "exact", 1.5e-3, 1.5e-2),
+                                                                  ("f32", "fast", 1.5e-3, 1.5e-2)])
+def test_adjoint_fp32_gradients_vs_fp64(orc, ctx, batch, gp, precision, median_bound, max_bound):
     """FP32 arithmetic against FP64 FD on 64 tracks.  Carrying the state in FP64, with the first two
     layers' reverse in FP64 (mixed, the default gradient precision), keeps every track within 1e-3; the
     chained fit's all-FP32 states carry the Kalman update's cancellation, so the all-FP32 gradient is
-    bounded per track only to ~1e-2."""
-    _, _, grads, _ = run_adjoint(ctx, *batch[1:], batch[0], gp)
+    bounded per track only to ~1e-2, in either arithmetic of the forward pass."""
+    _, _, grads, _ = run_adjoint(ctx, *batch[1:], batch[0], gp, precision)
     e = _errors(orc, grads, batch, "fd")
-    print(f"{gp} adjoint vs FP64 FD on 64 tracks: median {np.median(e):.2e} p90 {np.quantile(e, 0.9):.2e} "
-          f"max {e.max():.2e}", flush=True)
+    print(f"{gp} ({precision}) adjoint vs FP64 FD on 64 tracks: median {np.median(e):.2e} "
+          f"p90 {np.quantile(e, 0.9):.2e} max {e.max():.2e}", flush=True)
     assert np.median(e) < median_bound and e.max() < max_bound
 
 
 def test_checkpoint_stride_does_not_change_gradients(ctx, batch):
+    """Exact FP32: the segments' recomputed inputs are the forward pass's states bit for bit (no contraction,
+    so both kernels round every operation alike), and the gradients are the same words at every stride."""
     shape, trk, hits = batch
     ref = run_adjoint(ctx, trk, hits, shape, "f32", cs=4)[2]
     for cs in (1, 2, 8):
         g = run_adjoint(ctx, trk, hits, shape, "f32", cs=cs)[2]
         assert np.array_equal(g.view(np.uint64), ref.view(np.uint64)), cs
+
+
+@pytest.mark.parametrize("gp,precision,tol", [("f32", "fast", 1.5e-2), ("mixed", "exact", 1e-5), ("f64", "exact", 1e-8)])
+def test_checkpoint_stride_with_fused_arithmetic(ctx, batch, gp, precision, tol):
+    """Where the recompute fuses multiply-adds (fast FP32, and the FP64 states of MIXED and F64), the
+    compiler fuses them per kernel, so the forward pass and a segment's recompute may round a state
+    differently in its last bit; the gradients then agree across strides to the arithmetic's precision
+    (normwise per track), not word for word.  The chained fit's ill-conditioned updates amplify a last-bit
+    difference of an FP32 state to ~2e-3 of a gradient (measured), the scale of the FP32 gradient's own
+    error against FP64 (bounded at 1.5e-2 above); FP64 states differ by ~1e-12 to 2e-9."""
+    shape, trk, hits = batch
+    ref = run_adjoint(ctx, trk, hits, shape, gp, precision, cs=4)[2]
+    for cs in (1, 2, 8):
+        g = run_adjoint(ctx, trk, hits, shape, gp, precision, cs=cs)[2]
+        e = np.linalg.norm(g - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        print(f"{gp} ({precision}) stride {cs} vs 4: max per-track normwise difference {e.max():.2e}", flush=True)
+        assert e.max() < tol, cs
 
 
 @pytest.mark.parametrize("gp", ["f64", "mixed"])
