@@ -308,6 +308,24 @@ def test_exact_ragged_matches_oracle(orc, ctx, semantics, rounds):
 
 
 @pytest.mark.parametrize("rounds", [-1, -2, 1])
+@pytest.mark.parametrize("semantics", ["chained", "reference"])
+def test_fast_ragged_meets_the_fast_criterion(orc, orc_alt, ctx, semantics, rounds):
+    """Uneven batch with fast arithmetic, every schedule: each kernel is its own compilation of the fused
+    formulas, so the schedules are held to the fast-mode criterion against the masked oracle run (as close
+    to the IEEE-strict result as the reference's own fast-math build), not to each other's bits."""
+    counts, trk, hits, mask, offs = _ragged_inputs()
+    nblk = int(offs[-1])
+    sem = 0 if semantics == "reference" else 1
+    gpu, _ = run_ragged(ctx, trk, hits, mask, offs, 20, 32, semantics, precision="fast", rounds=rounds)
+    cpu = orc.run_f32(trk, hits, nblk, 20, 32, semantics=sem, mask=mask)
+    alt = orc_alt.run_f32(trk, hits, nblk, 20, 32, semantics=sem, mask=mask)
+    par64, cov64, _ = orc.run_f64(trk, hits, nblk, 20, 32, semantics=sem, mask=mask)
+    res = parity.compare_statistical(gpu, cpu, alt, 32, par64, cov64)
+    print(f"fast ragged {semantics} rounds={rounds}", json.dumps(res), flush=True)
+    assert res["ok"], res
+
+
+@pytest.mark.parametrize("rounds", [-1, -2, 1])
 @pytest.mark.parametrize("bsize", [1, 8, 64])
 def test_exact_ragged_other_widths(orc, ctx, bsize, rounds):
     """Matriplex widths other than 32 through both schedules (chained, chi2 on)."""
