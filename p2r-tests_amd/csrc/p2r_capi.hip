@@ -634,10 +634,14 @@ __global__ __launch_bounds__(64 * kPsWaves) void fwd_ragged_stream_kernel(const 
 constexpr int kAdjThreads = 256;
 
 template <class R, bool EXACT> struct MathOf;
+// fwd: one layer of the forward pass (the updated state and its chi2).  In FP32 it is p2r_forward's own
+// per-lane code (propagate_to_r, kalman_update), which the adjoint's forward pass and the reverse sweep's
+// segment recompute share; the FP64 instance computes it through layer_forward_values.
 template <> struct MathOf<float, true> {
   using T = exact::Trk; using H = exact::Hit;
   static __device__ __forceinline__ float fwd(const T& in, float k, const H& h, T& out) {
-    exact::LayerFwd f; exact::layer_forward_values(in, k, h, f, out); return exact::layer_chi2(f);
+    exact::propagate_to_r(in, k, h.pos[0], h.pos[1], out);
+    return exact::kalman_update(out, h);
   }
   static __device__ __forceinline__ void adj(const T& in, float k, const H& h, T& bar, float& kb, float hb[3]) {
     exact::layer_adjoint(in, k, h, bar, kb, hb);
@@ -646,7 +650,8 @@ template <> struct MathOf<float, true> {
 template <> struct MathOf<float, false> {
   using T = fast::Trk; using H = fast::Hit;
   static __device__ __forceinline__ float fwd(const T& in, float k, const H& h, T& out) {
-    fast::LayerFwd f; fast::layer_forward_values(in, k, h, f, out); return fast::layer_chi2(f);
+    fast::propagate_to_r(in, k, h.pos[0], h.pos[1], out);
+    return fast::kalman_update(out, h);
   }
   static __device__ __forceinline__ void adj(const T& in, float k, const H& h, T& bar, float& kb, float hb[3]) {
     fast::layer_adjoint(in, k, h, bar, kb, hb);
