@@ -634,9 +634,9 @@ __global__ __launch_bounds__(64 * kPsWaves) void fwd_ragged_stream_kernel(const 
 constexpr int kAdjThreads = 256;
 
 template <class R, bool EXACT> struct MathOf;
-// fwd: one layer of the forward pass (the updated state and its chi2).  In FP32 it is p2r_forward's own
-// per-lane code (propagate_to_r, kalman_update), which the adjoint's forward pass and the reverse sweep's
-// segment recompute share; the FP64 instance computes it through layer_forward_values.
+// fwd: one layer of the forward pass (the updated state and its chi2): p2r_forward's own per-lane code
+// (propagate_to_r, kalman_update) in the instance's arithmetic, shared by the adjoint's forward pass, the
+// reverse sweep's segment recompute and the FP64 tail.
 template <> struct MathOf<float, true> {
   using T = exact::Trk; using H = exact::Hit;
   static __device__ __forceinline__ float fwd(const T& in, float k, const H& h, T& out) {
@@ -660,7 +660,8 @@ template <> struct MathOf<float, false> {
 template <bool EXACT> struct MathOf<double, EXACT> {
   using T = dbl::Trk; using H = dbl::Hit;
   static __device__ __forceinline__ double fwd(const T& in, double k, const H& h, T& out) {
-    dbl::LayerFwd f; dbl::layer_forward_values(in, k, h, f, out); return dbl::layer_chi2(f);
+    dbl::propagate_to_r(in, k, h.pos[0], h.pos[1], out);
+    return dbl::kalman_update(out, h);
   }
   static __device__ __forceinline__ void adj(const T& in, double k, const H& h, T& bar, double& kb, double hb[3]) {
     dbl::layer_adjoint(in, k, h, bar, kb, hb);
