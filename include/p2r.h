@@ -162,7 +162,9 @@ int p2r_forward_ragged(p2r_ctx* ctx, const void* trk_in, const void* hits, void*
  *                 in FP64, each layer's reverse sweep in FP32 but for the
  *                 first two layers' (FP64), where the fit is least constrained
  *                 and the covariance update cancels most.
- * P2R_GRAD_F64:   everything in FP64. */
+ * P2R_GRAD_F64:   everything in FP64.
+ * The FP64 arithmetic divides, takes square roots and sin/cos within a few
+ * ulps (refined hardware estimates, its own reduction), not IEEE-rounded. */
 typedef enum p2r_grad_precision { P2R_GRAD_F32 = 0, P2R_GRAD_MIXED = 1, P2R_GRAD_F64 = 2 } p2r_grad_precision;
 
 /* Forward pass plus hand-written reverse-mode adjoint of the whole chained fit
