@@ -171,7 +171,7 @@ def test_fit_100x9600x20_stride4_outputs_match_oracle(orc, ctx, fit_100):
 def test_fit_100x9600x20_stride4_gradients(ctx, fit_100, fit_100_refs, gp, max_bound, median_bound):
     """FP64 within 1e-5 of FP64 central differences (their own truncation) on every sampled track.  The
     default gradient precision (mixed: FP64 state, FP32 reverse but for the first two layers') within 1e-3
-    on every sampled track, the north star's bar; all-FP32 carries the FP32 update's cancellation in its
+    on every sampled track, the gradient accuracy the adjoint is built to; all-FP32 carries the FP32 update's cancellation in its
     fitted states (a quarter of the tracks above 1e-3)."""
     shape, trk, hits, tracks = fit_100
     dual, fd = fit_100_refs
