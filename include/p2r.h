@@ -61,8 +61,10 @@ typedef struct p2r_shape {
  *   fused multiply-add, IEEE division and sqrt, FP64 determinant, sin/cos
  *   rounded from FP64); bit-identical to the IEEE-strict CPU build of the
  *   reference algorithm wherever the CPU's sinf/cosf are correctly rounded.
- * P2R_FAST: fused multiply-adds, hardware reciprocal/sqrt and an FP32 sin/cos;
- *   same algorithm, different FP32 rounding. */
+ * P2R_FAST: fused multiply-adds, hardware reciprocal/sqrt and an FP32 sin/cos
+ *   (within 2 units of 2^-24 of the true values for |angle| < 6000; larger
+ *   angles lose accuracy, NaN and infinities give NaN); same algorithm,
+ *   different FP32 rounding. */
 typedef enum p2r_precision { P2R_EXACT = 0, P2R_FAST = 1 } p2r_precision;
 
 /* kfact: curvature constant, 100/(-0.299792458*Bz) in the reference
