@@ -1108,8 +1108,9 @@ void launch_uniform_w(dim3 grid, dim3 block, hipStream_t st, const UniformLaunch
 template <int SEM, bool CHI2, bool EXACT>
 void launch_uniform_e(int waves, dim3 grid, dim3 block, hipStream_t st, const UniformLaunch& a, const float* in,
                       int l0, int l1, bool acc) {
-  // 0 = the library's choice: the 3-wave budget for the exact instance, 4 waves for the fast one (each
-  // the fastest measured on MI355X in both semantics)
+  // 0 = the library's choice: the 3-wave budget for the exact instance, 4 waves for the fast one.  The
+  // current kernels fit 5 or more waves per SIMD at the compiler's own allocation, and the three budgets
+  // compile to the same instructions; the choice stays for the autotuner.
   if (waves == 0) waves = EXACT ? 3 : 4;
   if (waves == 3) launch_uniform_w<SEM, CHI2, EXACT, 3>(grid, block, st, a, in, l0, l1, acc);
   else if (waves == 4) launch_uniform_w<SEM, CHI2, EXACT, 4>(grid, block, st, a, in, l0, l1, acc);
